@@ -1,0 +1,272 @@
+// gfx950 kernels of the reachable-pore labelling.
+//
+// The reference labels 6-connected components of "spans" (fastlib/src/cuda/VolumeCCL.cu:86-375):
+// scanning each x-row from a `background` previous value, every change of value toggles a
+// span open/closed; spans in neighbouring rows that overlap in x are connected; a component's
+// label is the minimum span-start linear index, then labels are renumbered 1..K in increasing
+// order of that index.  Two in-span voxels adjacent along x always share a span, so the
+// components are exactly the 6-connected components of the in-span voxel set, and the
+// renumbering orders components by their minimum linear voxel index.
+//
+// Here: (1) one thread per row computes the span set and points every in-span voxel at its
+// span start; (2) a lock-free union-find links rows across y and z, always towards the lower
+// index, so each component's root is its minimum index; (3) paths are flattened; (4) a
+// deterministic prefix count of roots gives every root its rank, i.e. the reference label.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cstdint>
+
+namespace ftc {
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr int kScanThreads = 256;
+constexpr int kScanPer = 16;
+constexpr int kScanChunk = kScanThreads * kScanPer;
+
+namespace {
+
+__device__ __forceinline__ uint32_t ld(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t find_root(const uint32_t* P, uint32_t x) {
+    uint32_t p = ld(P + x);
+    while (p != x) {
+        x = p;
+        p = ld(P + x);
+    }
+    return x;
+}
+
+__device__ __forceinline__ void merge(uint32_t* P, uint32_t a, uint32_t b) {
+    for (;;) {
+        a = find_root(P, a);
+        b = find_root(P, b);
+        if (a == b) return;
+        if (a < b) {
+            const uint32_t old = atomicMin(P + b, a);
+            if (old == b) return;
+            b = old;
+        } else {
+            const uint32_t old = atomicMin(P + a, b);
+            if (old == a) return;
+            a = old;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_spans(const uint8_t* __restrict__ mask, uint32_t* __restrict__ P, int nx,
+                                               int rows, uint8_t background) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= rows) return;
+    const long long base = (long long)row * nx;
+    uint8_t prev = background;
+    bool open = false;
+    uint32_t start = kNone;
+    for (int x = 0; x < nx; ++x) {
+        const uint8_t v = mask[base + x];
+        if (v != prev) {
+            open = !open;
+            if (open) start = (uint32_t)(base + x);
+        }
+        P[base + x] = open ? start : kNone;
+        prev = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_union(uint32_t* P, int nx, int ny, int nz) {
+    const long long n = (long long)nx * ny * nz;
+    const long long plane = (long long)nx * ny;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        if (ld(P + i) == kNone) continue;
+        const int y = (int)((i / nx) % ny);
+        const int z = (int)(i / plane);
+        // one link per span and row pair is enough, but linking every voxel keeps this simple
+        if (y > 0 && ld(P + i - nx) != kNone) merge(P, (uint32_t)i, (uint32_t)(i - nx));
+        if (z > 0 && ld(P + i - plane) != kNone) merge(P, (uint32_t)i, (uint32_t)(i - plane));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_flatten(uint32_t* P, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const uint32_t p = P[i];
+        if (p == kNone) continue;
+        P[i] = find_root(P, (uint32_t)i);
+    }
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+
+// exclusive block scan of one value per thread (kScanThreads threads); returns block total
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* excl) {
+    __shared__ uint32_t wtot[kScanThreads / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t inc = wave_incl_scan(v);
+    if (lane == 63) wtot[w] = inc;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (int i = 0; i < kScanThreads / 64; ++i) {
+        if (i < w) before += wtot[i];
+        total += wtot[i];
+    }
+    *excl = before + inc - v;
+    __syncthreads();
+    return total;
+}
+
+__device__ __forceinline__ bool is_root(const uint32_t* P, long long i) { return P[i] == (uint32_t)i; }
+
+__global__ __launch_bounds__(kScanThreads) void k_count_roots(const uint32_t* P, long long n, uint32_t* blocksum) {
+    const long long b0 = (long long)blockIdx.x * kScanChunk + (long long)threadIdx.x * kScanPer;
+    uint32_t c = 0;
+    for (int k = 0; k < kScanPer; ++k) {
+        const long long i = b0 + k;
+        if (i < n && is_root(P, i)) ++c;
+    }
+    uint32_t excl;
+    const uint32_t tot = block_excl_scan(c, &excl);
+    if (threadIdx.x == 0) blocksum[blockIdx.x] = tot;
+}
+
+// exclusive scan of nb block sums in one block; total number of roots to *total
+__global__ __launch_bounds__(1024) void k_scan_blocks(uint32_t* blocksum, int nb, uint32_t* total) {
+    __shared__ uint32_t part[1024];
+    const int per = (nb + 1023) / 1024;
+    const int b0 = threadIdx.x * per;
+    uint32_t s = 0;
+    for (int k = 0; k < per; ++k)
+        if (b0 + k < nb) s += blocksum[b0 + k];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int i = 0; i < 1024; ++i) {
+            const uint32_t t = part[i];
+            part[i] = run;
+            run += t;
+        }
+        *total = run;
+    }
+    __syncthreads();
+    uint32_t run = part[threadIdx.x];
+    for (int k = 0; k < per; ++k) {
+        if (b0 + k < nb) {
+            const uint32_t t = blocksum[b0 + k];
+            blocksum[b0 + k] = run;
+            run += t;
+        }
+    }
+}
+
+// roots get label = 1 + number of roots before them in linear order
+__global__ __launch_bounds__(kScanThreads) void k_assign_roots(const uint32_t* P, long long n, const uint32_t* blockoff,
+                                                               uint32_t* L) {
+    const long long b0 = (long long)blockIdx.x * kScanChunk + (long long)threadIdx.x * kScanPer;
+    uint32_t c = 0;
+    for (int k = 0; k < kScanPer; ++k) {
+        const long long i = b0 + k;
+        if (i < n && is_root(P, i)) ++c;
+    }
+    uint32_t excl;
+    (void)block_excl_scan(c, &excl);
+    uint32_t lab = blockoff[blockIdx.x] + excl;
+    for (int k = 0; k < kScanPer; ++k) {
+        const long long i = b0 + k;
+        if (i < n && is_root(P, i)) L[i] = ++lab;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_final_labels(const uint32_t* P, long long n, uint32_t* L) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const uint32_t p = P[i];
+        if (p == kNone)
+            L[i] = 0;
+        else if (p != (uint32_t)i)
+            L[i] = L[p];   // roots were labelled by the previous kernel and are not written here
+    }
+}
+
+// face table: row f, label l present on face f (Dir order X+,X-,Y+,Y-,Z+,Z-)
+__global__ __launch_bounds__(256) void k_face_table(const uint32_t* L, int nx, int ny, int nz, uint8_t* table,
+                                                    uint32_t numLabels) {
+    const int f = blockIdx.y;
+    const int k = f >> 1;
+    const int dims[3] = {nx, ny, nz};
+    const int fixed = (f & 1) ? 0 : dims[k] - 1;
+    const int s0 = (k + 1) % 3, s1 = (k + 2) % 3;
+    const long long m = (long long)dims[s0] * dims[s1];
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < m; q += (long long)gridDim.x * blockDim.x) {
+        int pos[3];
+        pos[k] = fixed;
+        pos[s0] = (int)(q % dims[s0]);
+        pos[s1] = (int)(q / dims[s0]);
+        const uint32_t l = L[(long long)pos[0] + (long long)nx * ((long long)pos[1] + (long long)ny * pos[2])];
+        if (l > 0) table[(size_t)f * numLabels + l] = 1;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_face_all(uint8_t* table, uint32_t numLabels) {
+    for (uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; l < numLabels; l += gridDim.x * blockDim.x) {
+        uint8_t a = 1;
+        for (int f = 0; f < 6; ++f) a &= table[(size_t)f * numLabels + l];
+        table[(size_t)6 * numLabels + l] = a;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_reachable(const uint32_t* L, long long n, const uint8_t* table, uint32_t numLabels,
+                                                   int fa, int fb, uint8_t* out) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const uint32_t l = L[i];
+        out[i] = (table[(size_t)fa * numLabels + l] && table[(size_t)fb * numLabels + l]) ? 255 : 0;
+    }
+}
+
+inline unsigned grid_for(long long n) {
+    long long b = (n + 255) / 256;
+    return (unsigned)(b > 65536 ? 65536 : (b < 1 ? 1 : b));
+}
+
+}  // namespace
+
+// Labels (0 = background, 1..K canonical) into L; returns K via *num_roots (device, 1 uint32).
+// P, L: n uint32 each; blocksum: ceil(n / 4096) uint32.
+hipError_t ccl_label(const uint8_t* mask, int nx, int ny, int nz, uint8_t background, uint32_t* P, uint32_t* L,
+                     uint32_t* blocksum, uint32_t* num_roots, hipStream_t s) {
+    const long long n = (long long)nx * ny * nz;
+    const int rows = ny * nz;
+    hipLaunchKernelGGL(k_spans, dim3((rows + 255) / 256), dim3(256), 0, s, mask, P, nx, rows, background);
+    hipLaunchKernelGGL(k_union, dim3(grid_for(n)), dim3(256), 0, s, P, nx, ny, nz);
+    hipLaunchKernelGGL(k_flatten, dim3(grid_for(n)), dim3(256), 0, s, P, n);
+    const int nb = (int)((n + kScanChunk - 1) / kScanChunk);
+    hipLaunchKernelGGL(k_count_roots, dim3(nb), dim3(kScanThreads), 0, s, (const uint32_t*)P, n, blocksum);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, blocksum, nb, num_roots);
+    hipLaunchKernelGGL(k_assign_roots, dim3(nb), dim3(kScanThreads), 0, s, (const uint32_t*)P, n,
+                       (const uint32_t*)blocksum, L);
+    hipLaunchKernelGGL(k_final_labels, dim3(grid_for(n)), dim3(256), 0, s, (const uint32_t*)P, n, L);
+    return hipGetLastError();
+}
+
+// 7 x numLabels face table (must be zeroed by the caller).
+hipError_t ccl_face_table(const uint32_t* L, int nx, int ny, int nz, uint8_t* table, uint32_t numLabels, hipStream_t s) {
+    const long long maxface = (long long)std::max(nx, ny) * std::max(std::max(ny, nz), nx) + (long long)nz * nx;
+    hipLaunchKernelGGL(k_face_table, dim3(grid_for(maxface), 6), dim3(256), 0, s, L, nx, ny, nz, table, numLabels);
+    hipLaunchKernelGGL(k_face_all, dim3(grid_for(numLabels)), dim3(256), 0, s, table, numLabels);
+    return hipGetLastError();
+}
+
+hipError_t ccl_reachable(const uint32_t* L, long long n, const uint8_t* table, uint32_t numLabels, int dir, uint8_t* out,
+                         hipStream_t s) {
+    const int k = dir / 2;
+    hipLaunchKernelGGL(k_reachable, dim3(grid_for(n)), dim3(256), 0, s, L, n, table, numLabels, 2 * k, 2 * k + 1, out);
+    return hipGetLastError();
+}
+
+}  // namespace ftc

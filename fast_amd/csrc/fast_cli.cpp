@@ -1,0 +1,252 @@
+// `fast tau` command line on top of libfasttau's C ABI.
+//
+// Mirrors the reference CLI (fast/src/main.cpp:20-134, fast/src/TauModule.h:12-164,
+// fast/src/VolumeIOModule.h, fast/src/CSVOutputModule.h): the same flags, the same
+// direction strings (fast/src/utility.h:84-113), one CSV row per direction with the same
+// key-sorted columns and default 6-significant-digit formatting, header only when the
+// output is new.  Additions: `-p mixed` (FP32 stencil / FP64 refinement), `--reach FILE`
+// (reachable-pore mask of the first direction's axis as .bin) and `--gpu N`.
+// `--cpu` is not offered: this library has no CPU solver.
+#include "fast_tau.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace {
+
+const char* kDirNames[6] = {"X_POS", "X_NEG", "Y_POS", "Y_NEG", "Z_POS", "Z_NEG"};
+
+std::vector<int> get_dirs(const std::string& s) {
+    if (s == "all") return {0, 1, 2, 3, 4, 5};
+    if (s == "pos") return {0, 2, 4};
+    if (s == "neg") return {1, 3, 5};
+    if (s == "xneg") return {1};
+    if (s == "yneg") return {3};
+    if (s == "zneg") return {5};
+    std::vector<int> d;
+    for (char c : s) {
+        if (c == 'x') d.push_back(0);
+        else if (c == 'y') d.push_back(2);
+        else if (c == 'z') d.push_back(4);
+    }
+    if (d.empty()) return {0};
+    return d;
+}
+
+// A CSV row: values kept as preformatted strings in a key-sorted map (std::map like the
+// reference's RunRow), numbers formatted by a default ostream.
+using Row = std::map<std::string, std::string>;
+template <class V>
+std::string fmt(const V& v) {
+    std::ostringstream o;
+    o << v;
+    return o.str();
+}
+
+class CsvOut {
+public:
+    explicit CsvOut(const std::string& path) : path_(path) {}
+    void add(const Row& row) {
+        bool is_new;
+        std::ostream* out = &std::cout;
+        if (!path_.empty()) {
+            std::ifstream probe(path_);
+            is_new = !probe.is_open();
+            if (!f_.is_open()) f_.open(path_, std::ios::app);
+            out = &f_;
+        } else {
+            is_new = (rows_ == 0);
+        }
+        if (is_new) {
+            size_t c = 0;
+            for (auto& kv : row) *out << kv.first << (++c != row.size() ? "," : "");
+            *out << "\n";
+        }
+        size_t c = 0;
+        for (auto& kv : row) *out << kv.second << (++c != row.size() ? "," : "");
+        *out << "\n";
+        out->flush();
+        ++rows_;
+    }
+
+private:
+    std::string path_;
+    std::ofstream f_;
+    int rows_ = 0;
+};
+
+void usage() {
+    std::cout << "Usage: fast tau <input.bin> [options]\n"
+                 "  -d, --dir STR        x|y|z (combinable), all, pos, neg, xneg, yneg, zneg (default x-)\n"
+                 "  --solver CG|BICGSTAB  (default CG)\n"
+                 "  --tol N              tolerance 1e-N (default 6)\n"
+                 "  --iter N             max iterations (default 10000)\n"
+                 "  -p, --precision P    double|float|mixed (default double)\n"
+                 "  -o, --output FILE    CSV output (append; header when new)\n"
+                 "  -v, --verbose\n"
+                 "  --over, --oversubscribe  accepted, no effect (288 GB HBM)\n"
+                 "  --reach FILE         write the reachable-pore mask (.bin) of the first direction\n"
+                 "  --gpu N              HIP device (default 0)\n";
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc < 2) {
+        usage();
+        return 0;
+    }
+    const std::string action = argv[1];
+    if (action == "-h" || action == "--help") {
+        usage();
+        return 0;
+    }
+    if (action != "tau") {
+        std::cerr << "Unknown action." << std::endl;
+        usage();
+        return 1;
+    }
+    std::string input, dirs = "x-", solver = "CG", prec = "double", output, reach;
+    int tol = 6, gpu = 0;
+    long iters = 10000;
+    bool verbose = false;
+    for (int i = 2; i < argc; ++i) {
+        const std::string a = argv[i];
+        auto next = [&](const char* what) -> std::string {
+            if (i + 1 >= argc) {
+                std::cerr << "missing value for " << what << std::endl;
+                std::exit(1);
+            }
+            return argv[++i];
+        };
+        if (a == "-h" || a == "--help") {
+            usage();
+            return 0;
+        } else if (a == "-d" || a == "--dir") {
+            dirs = next("--dir");
+        } else if (a == "--solver") {
+            solver = next("--solver");
+        } else if (a == "--tol") {
+            tol = std::atoi(next("--tol").c_str());
+        } else if (a == "--iter") {
+            iters = std::atol(next("--iter").c_str());
+        } else if (a == "-p" || a == "--precision") {
+            prec = next("--precision");
+        } else if (a == "-o" || a == "--output") {
+            output = next("--output");
+        } else if (a == "-v" || a == "--verbose") {
+            verbose = true;
+        } else if (a == "--over" || a == "--oversubscribe") {
+        } else if (a == "-j" || a == "--cputhreads") {
+            (void)next("--cputhreads");
+        } else if (a == "--cpu") {
+            std::cerr << "--cpu: this build has no CPU solver; the tau solve runs on the GPU" << std::endl;
+            return 1;
+        } else if (a == "--reach") {
+            reach = next("--reach");
+        } else if (a == "--gpu") {
+            gpu = std::atoi(next("--gpu").c_str());
+        } else if (!a.empty() && a[0] != '-' && input.empty()) {
+            input = a;
+        } else {
+            std::cerr << "unknown option " << a << std::endl;
+            return 1;
+        }
+    }
+    if (input.empty()) {
+        std::cerr << "Invalid input" << std::endl;
+        return 1;
+    }
+    int nx, ny, nz;
+    if (fast_load_bin_dims(input.c_str(), &nx, &ny, &nz) != FAST_OK) {
+        std::cerr << "Failed to load: " << input << std::endl;
+        return 1;
+    }
+    std::vector<uint8_t> mask((size_t)nx * ny * nz);
+    if (fast_load_bin(input.c_str(), mask.data(), mask.size()) != FAST_OK) {
+        std::cerr << "Failed to load: " << input << std::endl;
+        return 1;
+    }
+    fast_ctx* ctx = nullptr;
+    int rc = fast_ctx_create(gpu, 0, &ctx);
+    if (rc != FAST_OK) {
+        std::cerr << "no usable GPU (" << rc << ")" << std::endl;
+        return 1;
+    }
+    fast_volume* vol = nullptr;
+    rc = fast_volume_create(ctx, mask.data(), nx, ny, nz, &vol);
+    if (rc != FAST_OK) {
+        std::cerr << "volume upload failed: " << fast_ctx_last_error(ctx) << std::endl;
+        fast_ctx_destroy(ctx);
+        return 1;
+    }
+    uint64_t zeros = 0;
+    fast_volume_count_zero(vol, &zeros);
+    const float porosity = (float)((double)zeros / (double)mask.size());   // TauModule.h:62
+
+    fast_tau_params p;
+    fast_tau_params_default(&p);
+    p.solver = (solver == "BICGSTAB") ? FAST_BICGSTAB : FAST_CG;
+    const bool single = (prec == "float" || prec == "single" || prec == "f" || prec == "s");
+    const bool mixed = (prec == "mixed" || prec == "m");
+    p.precision = mixed ? FAST_MIXED : (single ? FAST_FP32 : FAST_FP64);
+    if (mixed) p.solver = FAST_BICGSTAB;
+    p.tolerance = std::pow(10.0, -tol);
+    p.max_iter = iters;
+    p.porosity = porosity;
+    p.verbose = verbose;
+
+    CsvOut out(output);
+    Row row;
+    row["ID"] = input;
+    row["precision"] = mixed ? "mixed" : (single ? "single" : "double");
+    row["device"] = "GPU";
+    row["dimx"] = fmt(nx);
+    row["dimy"] = fmt(ny);
+    row["dimz"] = fmt(nz);
+    const std::vector<int> dl = get_dirs(dirs);
+    int status = 0;
+    for (int d : dl) {
+        p.dir = d;
+        row["dir"] = kDirNames[d];
+        if (verbose) std::cout << "Tau Direction " << d << std::endl;
+        fast_tau_result r;
+        const auto t0 = std::chrono::system_clock::now();
+        rc = fast_tau_solve_volume(vol, &p, &r, nullptr);
+        const double dt = std::chrono::duration<double>(std::chrono::system_clock::now() - t0).count();
+        if (rc != FAST_OK && rc != FAST_E_NOCONV) {
+            std::cerr << "solve failed: " << fast_ctx_last_error(ctx) << std::endl;
+            status = 1;
+        }
+        row["tau"] = fmt(r.tau);
+        row["tauTime"] = fmt(dt);
+        row["tauSolver"] = (p.solver == FAST_BICGSTAB) ? "BICGSTAB" : "CG";
+        row["tauNU"] = "0";
+        row["tauTolerance"] = fmt(p.tolerance);
+        row["porosity"] = fmt((double)porosity);
+        row["tauIter"] = fmt((int)r.iterations);
+        if (verbose) std::cout << "Elapsed: " << dt << "s (" << dt / 60.0 << "m)" << std::endl;
+        out.add(row);
+    }
+    if (!reach.empty()) {
+        std::vector<uint8_t> rm(mask.size());
+        uint32_t nl = 0;
+        rc = fast_reachable_mask(vol, dl[0], 255, rm.data(), &nl);
+        if (rc != FAST_OK || fast_save_bin(reach.c_str(), rm.data(), nx, ny, nz) != FAST_OK) {
+            std::cerr << "reachable mask failed: " << fast_ctx_last_error(ctx) << std::endl;
+            status = 1;
+        }
+    }
+    fast_volume_destroy(vol);
+    fast_ctx_destroy(ctx);
+    return status;
+}

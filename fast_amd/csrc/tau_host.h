@@ -1,0 +1,81 @@
+// Host-side state of libfasttau (contexts, volumes, solve jobs).  Internal.
+#pragma once
+#include "fast_tau.h"
+#include "tau_internal.h"
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace fth {
+
+struct Shape {
+    int tx, ty, tz;
+};
+
+struct PinBuf {
+    void* ptr = nullptr;
+    size_t size = 0;
+};
+
+}  // namespace fth
+
+struct fast_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t budget = 0;
+    size_t held = 0;
+    std::string err;
+    int verbose = 0;
+    std::multimap<size_t, void*> freelist;     // cached device blocks by rounded size
+    std::unordered_map<void*, size_t> live;    // blocks in use
+    fth::PinBuf pin[4];
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    unsigned long long* zero_count = nullptr;
+    std::map<std::string, fth::Shape> tuned;   // autotuned tile shapes by size class
+    int chunk = 16;                            // Krylov iterations per polling round
+    bool use_graphs = true;                    // replay polling rounds from a captured HIP graph
+    std::mutex mu;
+};
+
+struct fast_volume {
+    fast_ctx* ctx = nullptr;
+    int nx = 0, ny = 0, nz = 0;
+    uint8_t* d_mask = nullptr;
+    uint8_t* d_code = nullptr;
+    uint32_t* d_labels = nullptr;   // cached CCL labels
+    uint32_t num_labels = 0;
+    int labels_bg = -1;             // background value the cached labels were computed for
+    uint64_t zeros = 0;
+};
+
+namespace fth {
+
+struct Job {
+    fast_volume* vol = nullptr;
+    fast_tau_params prm{};
+    fast_tau_result* res = nullptr;
+    double* conc = nullptr;
+    double porosity = 0;
+    double inner_tol = 0, outer_tol = 0;
+    int max_refine = 0;
+    int tz = 16;
+    Shape shape{64, 8, 16};
+};
+
+void set_error(fast_ctx* ctx, const char* fmt, ...);
+void* dalloc(fast_ctx* ctx, size_t bytes);
+void dfree(fast_ctx* ctx, void* p);
+void* pinned(fast_ctx* ctx, int slot, size_t bytes);
+bool hip_ok(fast_ctx* ctx, hipError_t e, const char* what);
+template <class T>
+ft::Coef<T> make_coef(int nx, int ny, int nz, int dir, double d_pore, double d_solid);
+std::string shape_key(int nx, int ny, int nz, int precision);
+Shape choose_shape(int nx, int ny, int nz, int precision, const fast_ctx* ctx);
+void load_tuning(fast_ctx* ctx);
+int solve_jobs(fast_ctx* ctx, std::vector<Job>& jobs);
+
+}  // namespace fth

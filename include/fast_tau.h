@@ -1,0 +1,205 @@
+/*
+ * fast_tau.h -- C ABI of libfasttau, the MI355X-native tortuosity (`fast tau`) solver.
+ *
+ * The reference exposes this path as a C++ template ABI of libfastlib.so:
+ *   template<T> T fast::getTortuosity(const Volume&, const TortuosityParams&,
+ *                                     DiffusionSolverType, Volume* conc, size_t* iters)
+ *       fastlib/include/fastlib/volume/VolumeMeasures.h:51-58,
+ *       fastlib/src/volume/VolumeMeasures.cpp:42-285
+ *   template<T> T fast::getPorosity(const Volume&)       VolumeMeasures.h:60-61
+ *   fast::getVolumeCCL / generateBoundaryConnectedVolume  VolumeSegmentation.h:33-47
+ *   fast::loadVolumeBinary / saveVolumeBinary              VolumeIO.h:15, VolumeIO.cpp:256-305
+ * Every entry point below names the reference interface it replaces.
+ *
+ * Conventions: return 0 on success, a negative FAST_E_* code on error; nothing
+ * calls exit(); no global mutable state (all device state lives in a fast_ctx);
+ * all caller buffers are host memory unless the name says _device.  One fast_ctx
+ * may be used by one host thread at a time; different contexts are independent.
+ * Volumes are uint8, x fastest then y then z (linearIndex, utility/Types.h:26-32);
+ * 0 = pore, any nonzero = solid (Diffusion.cpp:94-96).
+ */
+#ifndef FAST_TAU_H
+#define FAST_TAU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ---------------------------------------------------------- */
+#define FAST_OK 0
+#define FAST_E_BADARG (-1)   /* invalid argument / unsupported value */
+#define FAST_E_OOM (-2)      /* device memory (or HBM budget) exhausted */
+#define FAST_E_NOCONV (-3)   /* solver did not reach the tolerance (tau reported as 0) */
+#define FAST_E_HIP (-4)      /* HIP runtime error */
+#define FAST_E_NODEV (-5)    /* no usable GPU */
+#define FAST_E_IO (-6)       /* file I/O error */
+#define FAST_E_BUSY (-7)     /* server queue full / request not finished */
+
+/* ---- enums mirroring the reference ---------------------------------------- */
+/* Dir, fastlib/include/fastlib/utility/PrimitiveTypes.h:142-150 */
+enum fast_dir { FAST_X_POS = 0, FAST_X_NEG = 1, FAST_Y_POS = 2, FAST_Y_NEG = 3, FAST_Z_POS = 4, FAST_Z_NEG = 5 };
+/* DiffusionSolverType, VolumeMeasures.h:15-18 */
+enum fast_solver { FAST_BICGSTAB = 0, FAST_CG = 1 };
+/* Precision: FP64 and FP32 are the reference's `-p double|float` (TauModule.h:19,125-129);
+ * MIXED is this library's FP32-stencil / FP64-reduction / FP64 iterative-refinement form. */
+enum fast_precision { FAST_FP64 = 0, FAST_FP32 = 1, FAST_MIXED = 2 };
+/* Solve status (Solver<T>::Status, fastlib/include/fastlib/solver/Solver.h:47-51, extended) */
+enum fast_status { FAST_SOLVED = 0, FAST_NO_CONVERGENCE = 1, FAST_DIVERGED = 2, FAST_FAILED = 3 };
+
+/* TortuosityParams (VolumeMeasures.h:20-34) + SolveParams (Solver.h:39-43). */
+typedef struct fast_tau_params {
+    int dir;              /* fast_dir */
+    int solver;           /* fast_solver */
+    int precision;        /* fast_precision */
+    double tolerance;     /* relative residual, reference default 1e-6 (TauModule.h:16,137) */
+    long max_iter;        /* reference default 10000 (TauModule.h:17) */
+    double d_pore;        /* coeffs[0]; the reference CLI passes float 1.0f (TauModule.h:70) */
+    double d_solid;       /* coeffs[1]; the reference CLI passes float 0.001f */
+    double porosity;      /* < 0: computed on the device as float(#zero/N) (TauModule.h:62) */
+    double mixed_inner_tol; /* MIXED only: floor of the per-cycle FP32 relative residual reduction (0 = 1e-6);
+                               each cycle asks for half the remaining gap to the outer target */
+    double mixed_outer_tol; /* MIXED only: FP64 true relative residual target (0 = `tolerance`) */
+    int verbose;
+    int reserved[3];
+} fast_tau_params;
+
+/* Output of one solve: getTortuosity's return value and iterationsOut, plus status. */
+typedef struct fast_tau_result {
+    double tau;           /* 0 on failure / non-convergence, like the reference */
+    double porosity;      /* the porosity used in tau = porosity / (dc * N_primary) */
+    double dc;            /* mean boundary flux (VolumeMeasures.cpp:274) */
+    double rel_residual;  /* solver error: sqrt(|r|^2/|b|^2) of the recursive residual */
+    double true_rel_residual; /* MIXED: FP64 true residual at exit; otherwise -1 */
+    long iterations;      /* Krylov iterations (MIXED: total inner iterations) */
+    int status;           /* fast_status */
+    int restarts;         /* BiCGSTAB restarts (rho breakdown) */
+    int refinements;      /* MIXED: outer refinement cycles */
+    int reserved;
+    double solve_ms;      /* device time of the solve (events on the solve stream) */
+} fast_tau_result;
+
+/* Fill `p` with the reference CLI defaults: X_POS, CG, FP64, tol 1e-6, 10000 iterations,
+ * D = {1.0f, 0.001f}, porosity computed. */
+void fast_tau_params_default(fast_tau_params* p);
+
+/* ---- context --------------------------------------------------------------- */
+typedef struct fast_ctx fast_ctx;
+
+/* Create a context on HIP device `device`.  `hbm_budget` (bytes) bounds the device memory
+ * the context may hold (0 = 90% of free memory).  Replaces the reference's implicit global
+ * CUDA state (Diffusion.cu:8 __constant__ params, default stream). */
+int fast_ctx_create(int device, size_t hbm_budget, fast_ctx** out);
+void fast_ctx_destroy(fast_ctx* ctx);
+/* Last error message of this context (never NULL). */
+const char* fast_ctx_last_error(const fast_ctx* ctx);
+/* Bytes of device memory currently held by the context. */
+size_t fast_ctx_device_bytes(const fast_ctx* ctx);
+
+/* ---- device-resident volumes ---------------------------------------------- */
+/* A mask uploaded once and kept in HBM with its per-voxel stencil code; several solves
+ * (directions, precisions, solvers) reuse it.  Replaces Volume + Texture3DPtr::commit
+ * (Volume.cpp:19-43, DataPtr.cpp:209-242,377-407). */
+typedef struct fast_volume fast_volume;
+
+int fast_volume_create(fast_ctx* ctx, const uint8_t* mask, int nx, int ny, int nz, fast_volume** out);
+/* Same, from a mask already in device memory (not retained: it is copied). */
+int fast_volume_create_device(fast_ctx* ctx, const uint8_t* d_mask, int nx, int ny, int nz, fast_volume** out);
+void fast_volume_destroy(fast_volume* vol);
+/* #(mask == 0) on the device: Volume::sumZeroElems (Volume.cpp:261-272). */
+int fast_volume_count_zero(fast_volume* vol, uint64_t* count);
+
+/* ---- the drop-in ---------------------------------------------------------- */
+/* getPorosity<double> (VolumeMeasures.cpp:288-292) computed on the device. */
+int fast_porosity(fast_ctx* ctx, const uint8_t* mask, int nx, int ny, int nz, double* porosity);
+
+/* getTortuosity<T> (VolumeMeasures.cpp:42-285): assemble (matrix-free), solve, flux, tau.
+ * conc_out (nullable, N doubles) receives the concentration field (the reference's
+ * concetrationOutput).  Returns FAST_OK, FAST_E_NOCONV (result->tau = 0) or an error. */
+int fast_tau_solve(fast_ctx* ctx, const uint8_t* mask, int nx, int ny, int nz,
+                   const fast_tau_params* params, fast_tau_result* result, double* conc_out);
+/* Same on a device-resident volume (no host transfer of the mask). */
+int fast_tau_solve_volume(fast_volume* vol, const fast_tau_params* params, fast_tau_result* result,
+                          double* conc_out);
+
+/* Many solves in one persistent batched solve (uneven sizes allowed): every volume shares
+ * the same kernels, launched once per pass over the concatenated tile list and replayed
+ * from a HIP graph.  Per-volume results are bit-identical to solving the volume alone.
+ * params[i] applies to vols[i].  No reference counterpart: the reference solves volumes
+ * one by one (fast/src/SliceModule.h:126-133). */
+int fast_tau_solve_batch(fast_ctx* ctx, int count, fast_volume* const* vols,
+                         const fast_tau_params* params, fast_tau_result* results);
+
+/* ---- reachable pores (integer, bit-exact) ---------------------------------- */
+/* 6-connected labels of the reference's span set (background value `background`, 255 in
+ * the reference callers), numbered 1..K by each component's minimum linear index, 0 for
+ * background: VolumeCCL_Label (VolumeCCL.cu:381-620) via getVolumeCCL (VolumeSegmentation.cpp:33-64).
+ * labels_out: N uint32 (nullable); num_labels = K + 1. */
+int fast_ccl_labels(fast_volume* vol, uint8_t background, uint32_t* labels_out, uint32_t* num_labels);
+/* Reachable-pore mask for the axis of `dir`: 255 where the component touches both faces of
+ * that axis, else 0 (VolumeCCL::getDirMask(begin,end), VolumeSegmentation.cpp:14-24, and
+ * VolumeCCL_GenerateVolume, VolumeCCL.cu:821-860).  out: N bytes. */
+int fast_reachable_mask(fast_volume* vol, int dir, uint8_t background, uint8_t* out, uint32_t* num_labels);
+/* Per-face label table (VolumeCCL_BoundaryLabels, VolumeCCL.cu:753-818): 7 rows of
+ * num_labels bytes (rows in Dir order, row 6 = present on all faces).  table: 7*num_labels. */
+int fast_ccl_boundary_table(fast_volume* vol, uint8_t background, uint8_t* table, uint32_t table_labels);
+
+/* ---- serving: continuous batching ------------------------------------------ */
+/* A request loop that admits tau solves of mixed sizes by HBM budget, keeps masks and
+ * Krylov state resident, lets requests join the running batch at iteration boundaries
+ * and leave on convergence.  No reference counterpart. */
+typedef struct fast_server fast_server;
+typedef struct fast_server_config {
+    size_t hbm_budget;      /* bytes for resident masks + Krylov state (0 = context budget) */
+    int max_active;         /* max concurrently solving requests (0 = 64) */
+    int iters_per_round;    /* Krylov iterations between admission points (0 = 8) */
+    int reserved[4];
+} fast_server_config;
+
+int fast_server_create(fast_ctx* ctx, const fast_server_config* cfg, fast_server** out);
+/* Copy the mask and queue a request; returns a ticket.  Never blocks on the GPU. */
+int fast_server_submit(fast_server* srv, const uint8_t* mask, int nx, int ny, int nz,
+                       const fast_tau_params* params, uint64_t* ticket);
+/* Wait up to timeout_ms (negative = forever) for any finished request; returns FAST_E_BUSY
+ * on timeout.  latency_ms = completion - submission (host monotonic clock). */
+int fast_server_wait(fast_server* srv, int timeout_ms, uint64_t* ticket, fast_tau_result* result,
+                     double* latency_ms);
+/* Requests submitted and not yet returned by fast_server_wait. */
+int fast_server_pending(fast_server* srv);
+void fast_server_destroy(fast_server* srv);
+
+/* ---- input generation and .bin I/O (host only) ------------------------------- */
+/* Reference .bin format (VolumeIO.cpp:256-305): int32 PrimitiveType (UCHAR = 2), int32[3]
+ * dims, 1 byte flag, raw voxels. */
+int fast_load_bin_dims(const char* path, int* nx, int* ny, int* nz);
+int fast_load_bin(const char* path, uint8_t* mask, size_t capacity);
+int fast_save_bin(const char* path, const uint8_t* mask, int nx, int ny, int nz);
+
+/* Seeded generators (255 = solid, 0 = pore): bodies are added with centres uniform in the
+ * box, overlap allowed, until the solid fraction reaches `solid_fraction`; a voxel is solid
+ * when its centre (i + 1/2) lies inside a body.  Ellipsoid semi-axes are (a, a*r, a*r) with
+ * a uniformly random orientation (unit quaternion).  Returns the number of bodies placed. */
+long fast_generate_spheres(uint8_t* mask, int nx, int ny, int nz, double radius,
+                           double solid_fraction, uint64_t seed);
+long fast_generate_ellipsoids(uint8_t* mask, int nx, int ny, int nz, double a, double r,
+                              double solid_fraction, uint64_t seed);
+
+/* ---- diagnostics (test hooks on the same kernels the solver uses) ---------------- */
+/* y = A x with the matrix-free operator for direction `dir` (FP64). */
+int fast_debug_apply_f64(fast_volume* vol, int dir, const double* x, double* y);
+/* y = A x in FP32 (the operator of `-p float` and of the MIXED inner solve). */
+int fast_debug_apply_f32(fast_volume* vol, int dir, const float* x, float* y);
+/* Right-hand side b, initial guess x0 and Jacobi inverse 1/diag(A) (FP64). */
+int fast_debug_system_f64(fast_volume* vol, int dir, double* b, double* x0, double* invdiag);
+/* Mean device time (ms, HIP events on the solve stream) per launch of the three fused
+ * BiCGSTAB passes and per iteration, over `iters` iterations of a solve set up on `vol`
+ * (precision FAST_FP64, FAST_FP32, or FAST_MIXED for the FP32-vector / FP64-reduction
+ * inner passes).  ms: 4 doubles {pass1, pass2, pass3, iteration}. */
+int fast_debug_time_passes(fast_volume* vol, int dir, int precision, int iters, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

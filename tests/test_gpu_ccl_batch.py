@@ -1,0 +1,58 @@
+"""GPU reachable-pore labels (bit-exact) and batched solves (bit-identical to solo solves)."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import fast_amd as F
+from fast_amd import tau as T
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+def test_ccl_golden_cases(ctx):
+    for r in json.loads((GOLD / "ccl_cases.json").read_text()):
+        m = np.array(r["mask"], dtype=np.uint8).reshape(r["shape_zyx"])
+        vol = ctx.volume(m)
+        lab, nl = vol.ccl_labels(255)
+        assert nl == r["num_labels"], r["name"]
+        assert np.array_equal(lab.ravel(), np.array(r["labels"], dtype=np.uint32)), r["name"]
+        assert np.array_equal(vol.boundary_table(255).ravel(), np.array(r["table"], dtype=np.uint8)), r["name"]
+
+
+@pytest.mark.parametrize("shape,frac,seed", [((64, 64, 64), None, 0), ((50, 70, 90), 0.6, 1), ((33, 17, 129), 0.5, 2),
+                                             ((96, 96, 96), 0.69, 3)])
+def test_ccl_and_reachable_match_oracle(ctx, shape, frac, seed):
+    if frac is None:
+        m = F.generate_spheres(64, 6.0, 0.60, 0)   # the 64^3 sphere volume
+    else:
+        rng = np.random.default_rng(seed)
+        m = ((rng.random(shape) < frac) * 255).astype(np.uint8)
+    vol = ctx.volume(m)
+    lab, nl = vol.ccl_labels(255)
+    olab, onl = O.ccl(m)
+    assert nl == onl and np.array_equal(lab, olab)
+    for d in (0, 2, 4):
+        out, _ = vol.reachable_mask(d, 255)
+        oout, _, _ = O.reachable(m, d)
+        assert np.array_equal(out, oout), d
+
+
+def test_batch_equals_solo_bit_exact(ctx):
+    rng = np.random.default_rng(5)
+    vols, params = [], []
+    shapes = [(32, 48, 64), (80, 40, 48), (16, 16, 16), (48, 112, 96), (64, 64, 64)]
+    for i, (nx, ny, nz) in enumerate(shapes):
+        m = F.generate_ellipsoids((nx, ny, nz), 6.0, float(np.exp(rng.uniform(np.log(0.2), np.log(5)))), 0.55, 100 + i)
+        vols.append(ctx.volume(m))
+        params.append((i % 3) * 2)
+    for prec, solver in (("double", 0), ("double", 1), ("mixed", 0)):
+        ps = [T.make_params(d, solver, prec, 1e-6, 10000) for d in params]
+        batch = T.solve_batch(ctx, vols, ps)
+        for v, p, b in zip(vols, ps, batch):
+            solo, _ = v.solve(p)
+            assert solo.status == 0 and b.status == 0
+            assert solo.tau == b.tau and solo.iterations == b.iterations, (prec, solver, v.dims, solo.tau, b.tau)

@@ -1,0 +1,185 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle and the golden fixtures.
+
+Bars: the matrix-free operator, rhs, initial guess, Jacobi inverse and reachable-pore labels
+are bit-exact.  The FP64 and FP32 solves run the same Jacobi-BiCGSTAB / PCG recurrences as
+the oracle with order-independent (double-double, singly rounded) dot products, so their
+iteration counts and tau are bit-identical to the oracle's.  The mixed form (FP32 inner
+solve, FP64 refinement) agrees with the FP64 oracle to 1e-5 relative.
+"""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import fast_amd as F
+from fast_amd import tau as T
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLD = Path(__file__).resolve().parent / "golden"
+BICG, CG = 0, 1
+
+
+def rand_mask(shape, frac, seed, values=(255,)):
+    rng = np.random.default_rng(seed)
+    m = (rng.random(shape) < frac).astype(np.uint8)
+    vals = np.array(values, np.uint8)
+    return np.where(m, vals[rng.integers(0, len(vals), size=shape)], 0).astype(np.uint8)
+
+
+OPERATOR_CASES = [
+    ("config1_64", lambda: F.generate_spheres(64, 6.0, 0.60, 0)),
+    ("odd_37x23x19", lambda: rand_mask((19, 23, 37), 0.5, 1)),
+    ("tall_1x5x70", lambda: rand_mask((70, 5, 1), 0.4, 2)),
+    ("thin_67x3x1", lambda: rand_mask((1, 3, 67), 0.4, 3)),
+    ("line_64x1x1", lambda: rand_mask((1, 1, 64), 0.3, 4)),
+    ("nonbinary_20x18x9", lambda: rand_mask((9, 18, 20), 0.5, 5, values=(1, 7, 200, 255))),
+    ("wide_130x17x11", lambda: rand_mask((11, 17, 130), 0.45, 6)),
+]
+
+
+@pytest.mark.parametrize("name,make", OPERATOR_CASES, ids=[c[0] for c in OPERATOR_CASES])
+def test_operator_bit_exact(ctx, name, make):
+    m = make()
+    vol = ctx.volume(m)
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(m.shape)
+    for d in range(6):
+        y_gpu = vol.apply(d, x)
+        y_ref = O.apply(m, d, x)
+        assert np.array_equal(y_gpu, y_ref), (name, d, np.abs(y_gpu - y_ref).max())
+        xf = x.astype(np.float32)
+        assert np.array_equal(vol.apply(d, xf, "float"), O.apply(m, d, xf, "float")), (name, d, "fp32")
+
+
+@pytest.mark.parametrize("name,make", OPERATOR_CASES[:3], ids=[c[0] for c in OPERATOR_CASES[:3]])
+def test_system_bit_exact(ctx, name, make):
+    m = make()
+    vol = ctx.volume(m)
+    for d in range(6):
+        b, x0, inv = vol.system(d)
+        coef, rhs, ox0 = O.system(m, d)
+        assert np.array_equal(b, rhs), (name, d)
+        assert np.array_equal(x0, ox0), (name, d)
+        assert np.array_equal(inv, O.invdiag(m, d)), (name, d)
+
+
+def test_porosity_and_zero_count(ctx):
+    m = F.generate_spheres(64, 6.0, 0.60, 0)
+    vol = ctx.volume(m)
+    assert vol.count_zero() == O.count_zero(m)
+    assert T.getPorosity(m, ctx) == O.count_zero(m) / m.size
+
+
+@pytest.mark.parametrize("solver", [BICG, CG])
+def test_config1_tau_matches_oracle(ctx, solver):
+    """64^3 spheres r = 6, solid 0.60, seed 0, X_POS, FP64, tol 1e-6, max 10000 iterations."""
+    m = F.generate_spheres(64, 6.0, 0.60, 0)
+    ref, cref = O.tau(m, 0, solver=solver, tol=1e-6, conc=True)
+    vol = ctx.volume(m)
+    res, c = vol.solve(T.make_params(0, solver, "double", 1e-6, 10000), conc=True)
+    print(f"config1 solver={solver} gpu tau={res.tau!r} it={res.iterations} oracle tau={ref.tau!r} it={ref.iterations}")
+    assert res.status == 0 and ref.status == 0
+    assert res.iterations == ref.iterations
+    assert res.tau == ref.tau
+    assert np.array_equal(c, cref)
+
+
+@pytest.mark.parametrize("d", range(6))
+def test_tight_tolerance_all_directions(ctx, d):
+    m = rand_mask((24, 28, 32), 0.5, 7)
+    vol = ctx.volume(m)
+    for solver in (BICG, CG):
+        ref, _ = O.tau(m, d, solver=solver, tol=1e-11)
+        res, _ = vol.solve(T.make_params(d, solver, "double", 1e-11, 10000))
+        assert res.status == 0 and ref.status == 0
+        assert abs(res.tau - ref.tau) <= 1e-9 * ref.tau, (d, solver, res.tau, ref.tau)
+
+
+def test_tiny_golden_exact(ctx):
+    for r in json.loads((GOLD / "tiny_tau.json").read_text()):
+        m = np.array(r["mask"], dtype=np.uint8).reshape(r["shape_zyx"])
+        vol = ctx.volume(m)
+        for solver in (BICG, CG):
+            res, c = vol.solve(T.make_params(r["dir"], solver, "double", 1e-13, 10000), conc=True)
+            if np.isinf(r["tau"]):
+                assert np.isinf(res.tau), r["name"]
+                continue
+            assert abs(res.tau - r["tau"]) <= 1e-10 * r["tau"], (r["name"], r["dir"], solver, res.tau, r["tau"])
+            np.testing.assert_allclose(c.ravel(), np.array(r["conc"]), rtol=0, atol=1e-11)
+
+
+@pytest.mark.parametrize("precision", ["double", "float", "mixed"])
+def test_all_pore_is_one(ctx, precision):
+    m = np.zeros((40, 33, 50), np.uint8)
+    vol = ctx.volume(m)
+    for d in range(6):
+        solvers = (BICG,) if precision == "mixed" else (BICG, CG)
+        for solver in solvers:
+            tol = 1e-12 if precision == "double" else 1e-6
+            res, _ = vol.solve(T.make_params(d, solver, precision, tol, 10000))
+            assert res.status == 0, (precision, d, solver)
+            assert abs(res.tau - 1.0) < (1e-9 if precision == "double" else 1e-5), (precision, d, solver, res.tau)
+
+
+def test_parallel_slabs_closed_form(ctx):
+    m = np.zeros((30, 27, 45), np.uint8)
+    m[:, ::3, :] = 255
+    phi = (m == 0).mean()
+    expected = float(np.float32(phi)) / (phi + float(np.float32(0.001)) * (1 - phi))
+    vol = ctx.volume(m)
+    for solver in (BICG, CG):
+        res, _ = vol.solve(T.make_params(0, solver, "double", 1e-12, 10000))
+        assert abs(res.tau - expected) < 1e-9 * expected
+
+
+def test_fp32_dropin_matches_float_oracle(ctx):
+    m = F.generate_spheres(64, 6.0, 0.60, 0)
+    vol = ctx.volume(m)
+    for solver in (BICG, CG):
+        ref, _ = O.tau(m, 0, solver=solver, precision=1, tol=1e-6)
+        res, _ = vol.solve(T.make_params(0, solver, "float", 1e-6, 10000))
+        print(f"fp32 solver={solver} gpu={res.tau!r} ({res.iterations}) oracle={ref.tau!r} ({ref.iterations})")
+        assert res.status == 0 and ref.status == 0
+        assert res.iterations == ref.iterations and res.tau == ref.tau
+
+
+def test_mixed_matches_fp64_oracle(ctx):
+    m = F.generate_spheres(64, 6.0, 0.60, 0)
+    vol = ctx.volume(m)
+    for d in (0, 2, 4):
+        ref, _ = O.tau(m, d, solver=BICG, tol=1e-6)
+        res, _ = vol.solve(T.make_params(d, BICG, "mixed", 1e-6, 10000))
+        print(f"mixed d={d} tau={res.tau!r} it={res.iterations} ref={res.refinements} true_res={res.true_rel_residual:.3e}"
+              f" oracle={ref.tau!r} it={ref.iterations}")
+        assert res.status == 0
+        assert res.true_rel_residual <= 1e-6
+        assert abs(res.tau - ref.tau) <= 1e-5 * ref.tau
+
+
+def test_solve_is_deterministic(ctx):
+    m = rand_mask((40, 36, 44), 0.5, 8)
+    vol = ctx.volume(m)
+    for prec in ("double", "mixed"):
+        p = T.make_params(2, BICG, prec, 1e-6, 10000)
+        a, ca = vol.solve(p, conc=True)
+        b, cb = vol.solve(p, conc=True)
+        assert a.tau == b.tau and a.iterations == b.iterations and np.array_equal(ca, cb)
+
+
+def test_max_iter_reports_no_convergence(ctx):
+    m = F.generate_spheres(64, 6.0, 0.60, 0)
+    vol = ctx.volume(m)
+    res, _ = vol.solve(T.make_params(0, BICG, "double", 1e-6, 5))
+    assert res.status == 1 and res.tau == 0.0 and res.iterations == 5
+    assert T.getTortuosity(vol, T.TortuosityParams(dir=T.Dir.X_POS, maxIter=5)) == 0.0
+
+
+def test_get_tortuosity_mirror(ctx):
+    m = F.generate_spheres(48, 5.0, 0.5, 4)
+    its = []
+    tau = T.getTortuosity(m, T.TortuosityParams(dir=T.Dir.Y_NEG), T.DiffusionSolverType.DSOLVER_CG,
+                          iterationsOut=its, ctx=ctx)
+    ref, _ = O.tau(m, 3, solver=CG, tol=1e-6)
+    assert tau == ref.tau and its[0] == ref.iterations
