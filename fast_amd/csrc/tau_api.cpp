@@ -251,6 +251,7 @@ int fast_ctx_create(int device, size_t hbm_budget, fast_ctx** out) {
         return FAST_E_OOM;
     }
     load_tuning(ctx);
+    if (const char* e = std::getenv("FAST_TAU_MIXED_RESTART")) ctx->mixed_restart_cos = std::atof(e);
     *out = ctx;
     return FAST_OK;
 }
@@ -259,6 +260,9 @@ void fast_ctx_destroy(fast_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    // volumes still alive die with their context (their device memory is in `live`)
+    for (auto& kv : ctx->volumes) delete kv.first;
+    ctx->volumes.clear();
     for (auto& kv : ctx->live) (void)hipFree(kv.first);
     for (auto& kv : ctx->freelist) (void)hipFree(kv.second);
     for (auto& pb : ctx->pin)
@@ -307,6 +311,7 @@ static int volume_init(fast_ctx* ctx, const uint8_t* src, bool src_device, int n
         return FAST_E_HIP;
     }
     v->zeros = zeros;
+    ctx->volumes[v] = 1;
     *out = v;
     return FAST_OK;
 }
@@ -321,9 +326,11 @@ int fast_volume_create_device(fast_ctx* ctx, const uint8_t* d_mask, int nx, int 
 
 void fast_volume_destroy(fast_volume* v) {
     if (!v) return;
-    dfree(v->ctx, v->d_mask);
-    dfree(v->ctx, v->d_code);
-    dfree(v->ctx, v->d_labels);
+    fast_ctx* ctx = v->ctx;
+    dfree(ctx, v->d_mask);
+    dfree(ctx, v->d_code);
+    dfree(ctx, v->d_labels);
+    ctx->volumes.erase(v);
     delete v;
 }
 
@@ -360,8 +367,9 @@ struct Batch {
     Shape shape{64, 8, 16};
     bool mixed = false;
     std::vector<VolDesc> hvol;
+    using C = typename CmpOf<T, A>::type;
     std::vector<int> htile, hplane;
-    std::vector<Coef<T>> hcoef;
+    std::vector<Coef<C>> hcoef;
     std::vector<Coef<double>> hcoef64;
     std::vector<VolState<A>> hst;
     int ntiles = 0, nplanes = 0;
@@ -443,7 +451,7 @@ static int prepare(Batch<T, A>& B) {
         const int np = d.primary == 0 ? d.nx : (d.primary == 1 ? d.ny : d.nz);
         planes += np;
         total += align_up((size_t)v->nx * v->ny * v->nz, 64);
-        B.hcoef[i] = make_coef<T>(v->nx, v->ny, v->nz, j.prm.dir, j.prm.d_pore, j.prm.d_solid);
+        B.hcoef[i] = make_coef<typename Batch<T, A>::C>(v->nx, v->ny, v->nz, j.prm.dir, j.prm.d_pore, j.prm.d_solid);
         if (B.mixed) B.hcoef64[i] = make_coef<double>(v->nx, v->ny, v->nz, j.prm.dir, j.prm.d_pore, j.prm.d_solid);
         VolState<A>& s = B.hst[i];
         s.status = ST_RUNNING;
@@ -453,11 +461,11 @@ static int prepare(Batch<T, A>& B) {
                                   : A(1.1920928955078125e-07f) * A(1.1920928955078125e-07f);
         if (B.mixed) {
             s.mode = 1;
-            s.status = ST_DONE;   // the first refinement step starts every volume
-            s.tol = A(j.inner_tol);
-            s.tol_floor = A(j.inner_tol);
+            s.status = ST_FRESH;   // the first residual step starts every volume
             s.otol = j.outer_tol;
-            s.max_refine = j.max_refine;
+            s.delta2 = j.update_delta * j.update_delta;
+            s.restart_cos = ctx->mixed_restart_cos;
+            s.solver = j.prm.solver;
         }
     }
     B.ntiles = tiles;
@@ -475,7 +483,7 @@ static int prepare(Batch<T, A>& B) {
     const size_t o_vol = o;
     o = align_up(o + sizeof(VolDesc) * nv, 256);
     const size_t o_coef = o;
-    o = align_up(o + sizeof(Coef<T>) * nv, 256);
+    o = align_up(o + sizeof(Coef<typename Batch<T, A>::C>) * nv, 256);
     const size_t o_coef64 = o;
     o = align_up(o + sizeof(Coef<double>) * nv, 256);
     const size_t o_st = o;
@@ -496,7 +504,7 @@ static int prepare(Batch<T, A>& B) {
     if (!B.meta || !B.vecs) return FAST_E_OOM;
     std::vector<char> h(meta_bytes, 0);
     std::memcpy(h.data() + o_vol, B.hvol.data(), sizeof(VolDesc) * nv);
-    std::memcpy(h.data() + o_coef, B.hcoef.data(), sizeof(Coef<T>) * nv);
+    std::memcpy(h.data() + o_coef, B.hcoef.data(), sizeof(Coef<typename Batch<T, A>::C>) * nv);
     if (B.mixed) std::memcpy(h.data() + o_coef64, B.hcoef64.data(), sizeof(Coef<double>) * nv);
     std::memcpy(h.data() + o_st, B.hst.data(), sizeof(VolState<A>) * nv);
     std::memcpy(h.data() + o_tile, B.htile.data(), sizeof(int) * tiles);
@@ -511,7 +519,7 @@ static int prepare(Batch<T, A>& B) {
     B.L.nvol = nv;
     B.L.vol = (const VolDesc*)(m + o_vol);
     B.L.tile_vol = (const int*)(m + o_tile);
-    B.L.coef = (const Coef<T>*)(m + o_coef);
+    B.L.coef = (const Coef<typename Batch<T, A>::C>*)(m + o_coef);
     B.L.st = (VolState<A>*)(m + o_st);
     B.L.part = (double*)(m + o_part);
     B.coef64 = (Coef<double>*)(m + o_coef64);
@@ -558,6 +566,15 @@ static int drive(Batch<T, A>& B, int final_status, long max_chunks, Step step, R
             e = hipEventSynchronize(ctx->ev[prev]);
             if (!hip_ok(ctx, e, "poll")) return FAST_E_HIP;
             bool all_final = true, any_restart = false;
+            if (ctx->verbose) {
+                for (int i = 0; i < nv; ++i) {
+                    const VolState<A>& q = snap[prev][i];
+                    std::fprintf(stderr, "fasttau: round %ld vol %d status %d iter %d rsq %.3e tol2 %.3e r64sq %.3e "
+                                 "updates %d restarts %d rho %.3e alpha %.3e omega %.3e\n",
+                                 chunks - 1, i, q.status, q.iter, (double)q.rsq, (double)q.tol2, (double)q.r64sq,
+                                 q.refinements, q.restarts, (double)q.rho, (double)q.alpha, (double)q.omega);
+                }
+            }
             for (int i = 0; i < nv; ++i) {
                 const int st = snap[prev][i].status;
                 if (st == ST_RESTART) any_restart = true;
@@ -672,7 +689,10 @@ static int solve_plain(fast_ctx* ctx, std::vector<Job*>& jobs, Shape shape, int 
     return finish<T, A, T>(B, (const T*)B.L.vs.x, B.L.coef, ok);
 }
 
-static int solve_mixed(fast_ctx* ctx, std::vector<Job*>& jobs, Shape shape) {
+// Mixed form: FP32 BiCGSTAB with FP64 reductions and FP64 reliable updates.  After every
+// iteration an update step (no-op unless a volume asked for one) moves d into x64 and
+// replaces the residual by the FP64 true residual; the FP64 test ends the solve.
+static int solve_mixed(fast_ctx* ctx, std::vector<Job*>& jobs, Shape shape, int solver) {
     Batch<float, double> B;
     B.ctx = ctx;
     B.jobs = jobs;
@@ -682,25 +702,24 @@ static int solve_mixed(fast_ctx* ctx, std::vector<Job*>& jobs, Shape shape) {
     if (rc) return rc;
     hipStream_t s = ctx->stream;
     hipError_t e = hipEventRecord(ctx->t0, s);
+    if (e == hipSuccess) e = ft::mixed_update(B.L, s);
     if (e == hipSuccess) e = ft::mixed_residual(B.L, B.coef64, s);
     if (!hip_ok(ctx, e, "mixed init")) return FAST_E_HIP;
     long maxit = 0;
-    int maxref = 0;
-    for (auto* j : jobs) {
-        maxit = std::max(maxit, j->prm.max_iter);
-        maxref = std::max(maxref, j->max_refine);
-    }
+    for (auto* j : jobs) maxit = std::max(maxit, j->prm.max_iter);
     const int chunk = ctx->chunk;
-    const long max_chunks = 3 * ((long)(maxref + 1) * (maxit / chunk + 2)) + 16;
+    const long max_chunks = 3 * (maxit / chunk + 2) + 16;
     auto round = [&]() -> hipError_t {
         hipError_t e2 = hipSuccess;
-        for (int i = 0; i < chunk && e2 == hipSuccess; ++i) e2 = Passes<float, double>::bicg_iter(B.L, s);
-        if (e2 == hipSuccess) e2 = ft::mixed_update(B.L, s);
-        if (e2 == hipSuccess) e2 = ft::mixed_residual(B.L, B.coef64, s);
+        for (int i = 0; i < chunk && e2 == hipSuccess; ++i) {
+            e2 = (solver == FAST_BICGSTAB) ? Passes<float, double>::bicg_iter(B.L, s) : Passes<float, double>::cg_iter(B.L, s);
+            if (e2 == hipSuccess) e2 = ft::mixed_update(B.L, s);
+            if (e2 == hipSuccess) e2 = ft::mixed_residual(B.L, B.coef64, s);
+        }
         return e2;
     };
     auto step = [&]() -> hipError_t { return replay(B, round); };
-    auto restart = [&]() -> hipError_t { return hipSuccess; };   // inner breakdown ends the cycle on device
+    auto restart = [&]() -> hipError_t { return hipSuccess; };   // breakdowns restart on the device
     rc = drive(B, ST_IDLE, max_chunks, step, restart);
     if (rc) return rc;
     std::vector<bool> ok(jobs.size());
@@ -712,8 +731,8 @@ static int solve_mixed(fast_ctx* ctx, std::vector<Job*>& jobs, Shape shape) {
         ok[i] = success;
         j.res->rel_residual = err;
         j.res->true_rel_residual = err;
-        j.res->iterations = st.total_iter;
-        j.res->restarts = 0;
+        j.res->iterations = st.iter;
+        j.res->restarts = st.restarts;
         j.res->refinements = st.refinements;
         j.res->status = success ? FAST_SOLVED : FAST_NO_CONVERGENCE;
     }
@@ -723,6 +742,8 @@ static int solve_mixed(fast_ctx* ctx, std::vector<Job*>& jobs, Shape shape) {
 // Validate, fill defaults, group by (precision, solver, tile shape) and run each group as
 // one batched solve.
 int solve_jobs(fast_ctx* ctx, std::vector<Job>& jobs) {
+    ctx->verbose = 0;
+    for (auto& j : jobs) ctx->verbose |= j.prm.verbose;
     for (auto& j : jobs) {
         const fast_tau_params& p = j.prm;
         if (!j.vol || p.dir < 0 || p.dir > 5 || (p.solver != FAST_BICGSTAB && p.solver != FAST_CG) ||
@@ -730,16 +751,11 @@ int solve_jobs(fast_ctx* ctx, std::vector<Job>& jobs) {
             set_error(ctx, "invalid tau parameters");
             return FAST_E_BADARG;
         }
-        if (p.precision == FAST_MIXED && p.solver != FAST_BICGSTAB) {
-            set_error(ctx, "the mixed-precision form is a BiCGSTAB refinement");
-            return FAST_E_BADARG;
-        }
         const size_t n = (size_t)j.vol->nx * j.vol->ny * j.vol->nz;
         // porosity: precomputed value, else float(#zero / N) as fast/src/TauModule.h:62
         j.porosity = (p.porosity >= 0) ? p.porosity : (double)(float)((double)j.vol->zeros / (double)n);
-        j.inner_tol = p.mixed_inner_tol > 0 ? p.mixed_inner_tol : 1e-6;
+        j.update_delta = (p.mixed_inner_tol > 0 && p.mixed_inner_tol < 1) ? p.mixed_inner_tol : 0.1;
         j.outer_tol = p.mixed_outer_tol > 0 ? p.mixed_outer_tol : p.tolerance;
-        j.max_refine = 30;
         const Shape sh = choose_shape(j.vol->nx, j.vol->ny, j.vol->nz, p.precision, ctx);
         j.tz = sh.tz;
         j.shape = sh;
@@ -758,7 +774,7 @@ int solve_jobs(fast_ctx* ctx, std::vector<Job>& jobs) {
         else if (prec == FAST_FP32)
             rc = solve_plain<float, float>(ctx, g.second, sh, solver);
         else
-            rc = solve_mixed(ctx, g.second, sh);
+            rc = solve_mixed(ctx, g.second, sh, solver);
         if (rc) return rc;
     }
     for (auto& j : jobs)

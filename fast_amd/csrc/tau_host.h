@@ -38,6 +38,8 @@ struct fast_ctx {
     std::map<std::string, fth::Shape> tuned;   // autotuned tile shapes by size class
     int chunk = 16;                            // Krylov iterations per polling round
     bool use_graphs = true;                    // replay polling rounds from a captured HIP graph
+    double mixed_restart_cos = 1e-6;           // mixed form: restart when r0 and r are this close to orthogonal
+    std::unordered_map<fast_volume*, int> volumes;   // live volumes, released with the context
     std::mutex mu;
 };
 
@@ -60,8 +62,7 @@ struct Job {
     fast_tau_result* res = nullptr;
     double* conc = nullptr;
     double porosity = 0;
-    double inner_tol = 0, outer_tol = 0;
-    int max_refine = 0;
+    double update_delta = 0.1, outer_tol = 0;
     int tz = 16;
     Shape shape{64, 8, 16};
 };

@@ -8,7 +8,9 @@
 namespace ft {
 
 // Per-volume solve status kept on the device.
-enum : int { ST_RUNNING = 0, ST_DONE = 1, ST_RESTART = 2, ST_IDLE = 3 };
+// ST_UPDATE / ST_FRESH are the mixed form's requests for a reliable update (x64 += d and the
+// FP32 residual replaced by the FP64 true residual) or a fresh BiCGSTAB start from x64.
+enum : int { ST_RUNNING = 0, ST_DONE = 1, ST_RESTART = 2, ST_IDLE = 3, ST_UPDATE = 4, ST_FRESH = 5 };
 
 // Stencil coefficients of one (direction, precision) system.  The operator is matrix-free:
 // a face between voxels of solid flags (o, n) has coefficient ci[o+n]; a box face on the
@@ -40,17 +42,19 @@ struct VolState {
     A rho, rho_old, alpha, omega, beta;
     A r0sq, bsq, tol2, rsq, absnew;
     A tol, thresh, eps2;
-    A tol_floor;                   // mixed: smallest inner relative tolerance asked for
-    // mixed refinement (FP64 outer loop)
-    double r64sq, b64sq, otol;     // last FP64 |r|^2, |b|^2, outer relative tolerance
-    long long maxiter;             // Krylov iteration cap (inner cap in mixed mode)
-    long long total_iter;          // mixed: inner iterations summed over refinement cycles
+    // mixed form (FP32 Krylov process with FP64 reliable updates)
+    double r64sq, b64sq, otol;     // last FP64 |r|^2, |b|^2, relative tolerance on the true residual
+    double rsq_ref, delta2;        // |r|^2 at the last update; update when |r|^2 < delta2 * rsq_ref
+    double restart_cos;            // mixed: fresh start when |r0.r| < restart_cos |r0| |r|
+    long long maxiter;             // Krylov iteration cap
     int iter, status, restarts, first;
     int phase;                     // ping-pong parity counter (never reset)
     int zero_rhs;
-    int mode;                      // 0 = plain solve, 1 = inner solve of mixed refinement
-    int refinements, max_refine;
-    int pad0;
+    int mode;                      // 0 = plain solve, 1 = mixed form
+    int refinements;               // mixed: FP64 residual replacements so far
+    int discard;                   // mixed: drop d (non-finite) at the next update
+    int reshadow;                  // mixed: the next update also resets r0 = r (Eigen's restart)
+    int solver;                    // mixed: 0 = BiCGSTAB, 1 = CG
 };
 
 // Vectors of one (batched) solve, concatenated over its volumes (element offset VolDesc::off).
@@ -64,14 +68,29 @@ struct VecSet {
     double* x64;   // mixed refinement: FP64 solution
 };
 
+// Arithmetic type of a pass: the storage type, except in the mixed form (FP32 vectors,
+// FP64 accumulation) where every value is widened to FP64 after the load and rounded to FP32
+// only on the store.  The row sum of the 7-point operator cancels by a factor of order N^2 on
+// smooth fields, so FP32 arithmetic (or FP32-rounded coefficients) would perturb A x by
+// ~1e-3 relative at 256^3 and break BiCGSTAB; with FP64 arithmetic only storage rounds.
+template <class T, class A>
+struct CmpOf {
+    using type = T;
+};
+template <>
+struct CmpOf<float, double> {
+    using type = double;
+};
+
 // Launch description shared by all passes of one batched solve.
 template <class T, class A>
 struct Launch {
+    using C = typename CmpOf<T, A>::type;
     int tx, ty;               // block shape: one of the compiled shapes
     int ntiles, nvol;
     const VolDesc* vol;       // device
     const int* tile_vol;      // device: tile -> volume
-    const Coef<T>* coef;      // device: per volume
+    const Coef<C>* coef;      // device: per volume, in the arithmetic type
     VolState<A>* st;          // device: per volume
     double* part;             // device: ntiles * 8 partial sums ((hi, lo) pairs)
     VecSet vs;
@@ -99,12 +118,15 @@ struct Passes {
     static hipError_t apply(const Launch<T, A>&, const T* x, T* y, hipStream_t);
 };
 
-// Mixed precision.  The inner system is FP32 (T = float, A = double); the outer residual
-// is FP64 on x64 with the FP64 coefficients coef64 (per volume).  For every volume in
-// ST_DONE: on its first refinement step x64 = x0; then r64 = b - A x64, the outer test, and
-// (unless converged) a fresh inner BiCGSTAB on d with r = r0 = float(r64), d = p = v = 0.
+// Mixed form.  One BiCGSTAB process runs with FP32 vectors and stencil (T = float) and FP64
+// reductions and scalars (A = double) on the correction d; x64 accumulates the solution in
+// FP64.  For every volume in ST_UPDATE or ST_FRESH:
+//   mixed_update:   x64 += d (unless discarded), d = 0;
+//   mixed_residual: x64 = x0 on the first step; r64 = b - A x64 with the FP64 operator
+//                   (coef64); r = float(r64) replaces the recursive residual; on ST_FRESH
+//                   also r0 = r, p = v = 0.  Then the FP64 test |r64| <= otol |b| and either
+//                   ST_IDLE (finished) or ST_RUNNING with the BiCGSTAB scalars carried over.
 hipError_t mixed_residual(const Launch<float, double>&, const Coef<double>* coef64, hipStream_t);
-// x64 += d for volumes whose inner solve finished (status ST_DONE).
 hipError_t mixed_update(const Launch<float, double>&, hipStream_t);
 
 // Flux integral of the concentration `conc` (element offsets VolDesc::off):

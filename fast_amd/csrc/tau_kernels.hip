@@ -272,9 +272,10 @@ struct Pt {
 // resolved up front (a dynamically indexed array would live in scratch memory).
 template <class T, class A>
 struct PassCtx {
-    Coef<T> c;
+    using C = typename CmpOf<T, A>::type;
+    Coef<C> c;
     int primary, sgn, np, first;
-    T alpha, beta, omega;
+    C alpha, beta, omega;
     const uint8_t* code;
     T* x;
     T* r;
@@ -292,28 +293,29 @@ struct PassOps;
 // INIT_BICG: u = x0; r = b - A x0, x = x0, r0 = r, p = v = 0
 template <class T, class A>
 struct PassOps<T, A, K_INIT_BICG> {
+    using C = typename CmpOf<T, A>::type;
     static constexpr int NP = 2;   // |r|^2, |b|^2
-    template <class C>
-    __device__ static Raw<T> fetch(const C& pc, long long idx, long long lin, bool centre) {
-        Raw<T> w;
+    template <class X>
+    __device__ static Raw<C> fetch(const X& pc, long long idx, long long lin, bool centre) {
+        Raw<C> w;
         w.code = pc.code[lin];
         return w;
     }
-    template <class C>
-    __device__ static Pt<T> compute(const C& pc, const Raw<T>& w, int bx, int k) {
-        Pt<T> p;
+    template <class X>
+    __device__ static Pt<C> compute(const X& pc, const Raw<C>& w, int bx, int k) {
+        Pt<C> p;
         p.code = w.code;
-        p.u = x0_of<T>(k, pc.np, pc.sgn);
-        p.dg = diag_of<T>(w.code, bx, pc.primary, pc.c);
+        p.u = x0_of<C>(k, pc.np, pc.sgn);
+        p.dg = diag_of<C>(w.code, bx, pc.primary, pc.c);
         return p;
     }
-    template <class C, class Acc>
-    __device__ static void epi(const C& pc, long long idx, int k, T Au, const Pt<T>& pt, Acc (&acc)[NP]) {
-        const T b = rhs_of<T>(pt.code, k, pc.np, pc.c);
-        const T r = b - Au;
-        pc.x[idx] = pt.u;
-        pc.r[idx] = r;
-        pc.r0[idx] = r;
+    template <class X, class Acc>
+    __device__ static void epi(const X& pc, long long idx, int k, C Au, const Pt<C>& pt, Acc (&acc)[NP]) {
+        const C b = rhs_of<C>(pt.code, k, pc.np, pc.c);
+        const C r = b - Au;
+        pc.x[idx] = T(pt.u);
+        pc.r[idx] = T(r);
+        pc.r0[idx] = T(r);
         pc.p_old[idx] = T(0);
         pc.v_old[idx] = T(0);
         acc[0].add(r, r);
@@ -324,28 +326,29 @@ struct PassOps<T, A, K_INIT_BICG> {
 // INIT_CG: u = x0; r = b - A x0, x = x0
 template <class T, class A>
 struct PassOps<T, A, K_INIT_CG> {
+    using C = typename CmpOf<T, A>::type;
     static constexpr int NP = 3;   // |r|^2, |b|^2, r.(M^-1 r)
-    template <class C>
-    __device__ static Raw<T> fetch(const C& pc, long long idx, long long lin, bool centre) {
-        Raw<T> w;
+    template <class X>
+    __device__ static Raw<C> fetch(const X& pc, long long idx, long long lin, bool centre) {
+        Raw<C> w;
         w.code = pc.code[lin];
         return w;
     }
-    template <class C>
-    __device__ static Pt<T> compute(const C& pc, const Raw<T>& w, int bx, int k) {
-        Pt<T> p;
+    template <class X>
+    __device__ static Pt<C> compute(const X& pc, const Raw<C>& w, int bx, int k) {
+        Pt<C> p;
         p.code = w.code;
-        p.u = x0_of<T>(k, pc.np, pc.sgn);
-        p.dg = diag_of<T>(w.code, bx, pc.primary, pc.c);
+        p.u = x0_of<C>(k, pc.np, pc.sgn);
+        p.dg = diag_of<C>(w.code, bx, pc.primary, pc.c);
         return p;
     }
-    template <class C, class Acc>
-    __device__ static void epi(const C& pc, long long idx, int k, T Au, const Pt<T>& pt, Acc (&acc)[NP]) {
-        const T b = rhs_of<T>(pt.code, k, pc.np, pc.c);
-        const T r = b - Au;
-        const T inv = T(1) / pt.dg;
-        pc.x[idx] = pt.u;
-        pc.r[idx] = r;
+    template <class X, class Acc>
+    __device__ static void epi(const X& pc, long long idx, int k, C Au, const Pt<C>& pt, Acc (&acc)[NP]) {
+        const C b = rhs_of<C>(pt.code, k, pc.np, pc.c);
+        const C r = b - Au;
+        const C inv = C(1) / pt.dg;
+        pc.x[idx] = T(pt.u);
+        pc.r[idx] = T(r);
         acc[0].add(r, r);
         acc[1].add(b, b);
         acc[2].add(r, inv * r);
@@ -355,28 +358,29 @@ struct PassOps<T, A, K_INIT_CG> {
 // RESTART: u = x; r = b - A x, r0 = r
 template <class T, class A>
 struct PassOps<T, A, K_RESTART> {
+    using C = typename CmpOf<T, A>::type;
     static constexpr int NP = 1;
-    template <class C>
-    __device__ static Raw<T> fetch(const C& pc, long long idx, long long lin, bool centre) {
-        Raw<T> w;
-        w.a = pc.x[idx];
+    template <class X>
+    __device__ static Raw<C> fetch(const X& pc, long long idx, long long lin, bool centre) {
+        Raw<C> w;
+        w.a = C(pc.x[idx]);
         w.code = pc.code[lin];
         return w;
     }
-    template <class C>
-    __device__ static Pt<T> compute(const C& pc, const Raw<T>& w, int bx, int k) {
-        Pt<T> p;
+    template <class X>
+    __device__ static Pt<C> compute(const X& pc, const Raw<C>& w, int bx, int k) {
+        Pt<C> p;
         p.code = w.code;
         p.u = w.a;
-        p.dg = diag_of<T>(w.code, bx, pc.primary, pc.c);
+        p.dg = diag_of<C>(w.code, bx, pc.primary, pc.c);
         return p;
     }
-    template <class C, class Acc>
-    __device__ static void epi(const C& pc, long long idx, int k, T Au, const Pt<T>& pt, Acc (&acc)[NP]) {
-        const T b = rhs_of<T>(pt.code, k, pc.np, pc.c);
-        const T r = b - Au;
-        pc.r[idx] = r;
-        pc.r0[idx] = r;
+    template <class X, class Acc>
+    __device__ static void epi(const X& pc, long long idx, int k, C Au, const Pt<C>& pt, Acc (&acc)[NP]) {
+        const C b = rhs_of<C>(pt.code, k, pc.np, pc.c);
+        const C r = b - Au;
+        pc.r[idx] = T(r);
+        pc.r0[idx] = T(r);
         acc[0].add(r, r);
     }
 };
@@ -384,57 +388,60 @@ struct PassOps<T, A, K_RESTART> {
 // APPLY (diagnostic): y = A x
 template <class T, class A>
 struct PassOps<T, A, K_APPLY> {
+    using C = typename CmpOf<T, A>::type;
     static constexpr int NP = 0;
-    template <class C>
-    __device__ static Raw<T> fetch(const C& pc, long long idx, long long lin, bool centre) {
-        Raw<T> w;
-        w.a = pc.x[idx];
+    template <class X>
+    __device__ static Raw<C> fetch(const X& pc, long long idx, long long lin, bool centre) {
+        Raw<C> w;
+        w.a = C(pc.x[idx]);
         w.code = pc.code[lin];
         return w;
     }
-    template <class C>
-    __device__ static Pt<T> compute(const C& pc, const Raw<T>& w, int bx, int k) {
-        Pt<T> p;
+    template <class X>
+    __device__ static Pt<C> compute(const X& pc, const Raw<C>& w, int bx, int k) {
+        Pt<C> p;
         p.code = w.code;
         p.u = w.a;
-        p.dg = diag_of<T>(w.code, bx, pc.primary, pc.c);
+        p.dg = diag_of<C>(w.code, bx, pc.primary, pc.c);
         return p;
     }
-    template <class C, class Acc>
-    __device__ static void epi(const C& pc, long long idx, int k, T Au, const Pt<T>& pt, Acc (&)[1]) {
-        pc.t[idx] = Au;
+    template <class X, class Acc>
+    __device__ static void epi(const X& pc, long long idx, int k, C Au, const Pt<C>& pt, Acc (&)[1]) {
+        pc.t[idx] = T(Au);
     }
 };
 
 // BICG1: p = r + beta (p_old - omega v_old); u = M^-1 p; v = A u; partial r0.v
 template <class T, class A>
 struct PassOps<T, A, K_BICG1> {
+    using C = typename CmpOf<T, A>::type;
     static constexpr int NP = 1;
-    template <class C>
-    __device__ static Raw<T> fetch(const C& pc, long long idx, long long lin, bool centre) {
-        Raw<T> w;
-        w.a = pc.r[idx];
-        w.b = pc.p_old[idx];
-        w.c = pc.v_old[idx];
-        if (centre) w.e = pc.r0[idx];
+    template <class X>
+    __device__ static Raw<C> fetch(const X& pc, long long idx, long long lin, bool centre) {
+        Raw<C> w;
+        w.a = C(pc.r[idx]);
+        w.b = C(pc.p_old[idx]);
+        w.c = C(pc.v_old[idx]);
+        if (centre) w.e = C(pc.r0[idx]);
         w.code = pc.code[lin];
         return w;
     }
-    template <class C>
-    __device__ static Pt<T> compute(const C& pc, const Raw<T>& w, int bx, int k) {
-        Pt<T> q;
+    template <class X>
+    __device__ static Pt<C> compute(const X& pc, const Raw<C>& w, int bx, int k) {
+        Pt<C> q;
         q.code = w.code;
-        const T p = w.a + pc.beta * (w.b - pc.omega * w.c);
-        q.dg = diag_of<T>(w.code, bx, pc.primary, pc.c);
-        q.u = (T(1) / q.dg) * p;
+        // p as it will be stored: the stencil and the x update must see the same direction
+        const C p = C(T(w.a + pc.beta * (w.b - pc.omega * w.c)));
+        q.dg = diag_of<C>(w.code, bx, pc.primary, pc.c);
+        q.u = (C(1) / q.dg) * p;
         q.aux = p;
         q.e = w.e;
         return q;
     }
-    template <class C, class Acc>
-    __device__ static void epi(const C& pc, long long idx, int k, T Au, const Pt<T>& pt, Acc (&acc)[NP]) {
-        pc.p_new[idx] = pt.aux;
-        pc.v_new[idx] = Au;
+    template <class X, class Acc>
+    __device__ static void epi(const X& pc, long long idx, int k, C Au, const Pt<C>& pt, Acc (&acc)[NP]) {
+        pc.p_new[idx] = T(pt.aux);
+        pc.v_new[idx] = T(Au);
         acc[0].add(pt.e, Au);
     }
 };
@@ -442,28 +449,29 @@ struct PassOps<T, A, K_BICG1> {
 // BICG2: s = r - alpha v; u = M^-1 s; t = A u; partials t.s, t.t
 template <class T, class A>
 struct PassOps<T, A, K_BICG2> {
+    using C = typename CmpOf<T, A>::type;
     static constexpr int NP = 2;
-    template <class C>
-    __device__ static Raw<T> fetch(const C& pc, long long idx, long long lin, bool centre) {
-        Raw<T> w;
-        w.a = pc.r[idx];
-        w.b = pc.v_new[idx];
+    template <class X>
+    __device__ static Raw<C> fetch(const X& pc, long long idx, long long lin, bool centre) {
+        Raw<C> w;
+        w.a = C(pc.r[idx]);
+        w.b = C(pc.v_new[idx]);
         w.code = pc.code[lin];
         return w;
     }
-    template <class C>
-    __device__ static Pt<T> compute(const C& pc, const Raw<T>& w, int bx, int k) {
-        Pt<T> q;
+    template <class X>
+    __device__ static Pt<C> compute(const X& pc, const Raw<C>& w, int bx, int k) {
+        Pt<C> q;
         q.code = w.code;
-        const T s = w.a - pc.alpha * w.b;
-        q.dg = diag_of<T>(w.code, bx, pc.primary, pc.c);
-        q.u = (T(1) / q.dg) * s;
+        const C s = w.a - pc.alpha * w.b;
+        q.dg = diag_of<C>(w.code, bx, pc.primary, pc.c);
+        q.u = (C(1) / q.dg) * s;
         q.aux = s;
         return q;
     }
-    template <class C, class Acc>
-    __device__ static void epi(const C& pc, long long idx, int k, T Au, const Pt<T>& pt, Acc (&acc)[NP]) {
-        pc.t[idx] = Au;
+    template <class X, class Acc>
+    __device__ static void epi(const X& pc, long long idx, int k, C Au, const Pt<C>& pt, Acc (&acc)[NP]) {
+        pc.t[idx] = T(Au);
         acc[0].add(Au, pt.aux);
         acc[1].add(Au, Au);
     }
@@ -472,28 +480,30 @@ struct PassOps<T, A, K_BICG2> {
 // CG1: z = M^-1 r; p = z (first iteration) or z + beta p_old; q = A p; partial p.q
 template <class T, class A>
 struct PassOps<T, A, K_CG1> {
+    using C = typename CmpOf<T, A>::type;
     static constexpr int NP = 1;
-    template <class C>
-    __device__ static Raw<T> fetch(const C& pc, long long idx, long long lin, bool centre) {
-        Raw<T> w;
-        w.a = pc.r[idx];
-        if (!pc.first) w.b = pc.p_old[idx];
+    template <class X>
+    __device__ static Raw<C> fetch(const X& pc, long long idx, long long lin, bool centre) {
+        Raw<C> w;
+        w.a = C(pc.r[idx]);
+        if (!pc.first) w.b = C(pc.p_old[idx]);
         w.code = pc.code[lin];
         return w;
     }
-    template <class C>
-    __device__ static Pt<T> compute(const C& pc, const Raw<T>& w, int bx, int k) {
-        Pt<T> q;
+    template <class X>
+    __device__ static Pt<C> compute(const X& pc, const Raw<C>& w, int bx, int k) {
+        Pt<C> q;
         q.code = w.code;
-        q.dg = diag_of<T>(w.code, bx, pc.primary, pc.c);
-        const T z = (T(1) / q.dg) * w.a;
-        q.u = pc.first ? z : z + pc.beta * w.b;
+        q.dg = diag_of<C>(w.code, bx, pc.primary, pc.c);
+        const C z = (C(1) / q.dg) * w.a;
+        // p as it will be stored: the stencil and the x update must see the same direction
+        q.u = C(T(pc.first ? z : z + pc.beta * w.b));
         return q;
     }
-    template <class C, class Acc>
-    __device__ static void epi(const C& pc, long long idx, int k, T Au, const Pt<T>& pt, Acc (&acc)[NP]) {
-        pc.p_new[idx] = pt.u;
-        pc.t[idx] = Au;
+    template <class X, class Acc>
+    __device__ static void epi(const X& pc, long long idx, int k, C Au, const Pt<C>& pt, Acc (&acc)[NP]) {
+        pc.p_new[idx] = T(pt.u);
+        pc.t[idx] = T(Au);
         acc[0].add(pt.u, Au);
     }
 };
@@ -607,7 +617,8 @@ __global__ __launch_bounds__(TX* TY) void k_stencil(Launch<T, A> L, const T* xin
     using Ops = PassOps<T, A, KIND>;
     constexpr int NP = Ops::NP;
     constexpr int NPA = NP > 0 ? NP : 1;
-    __shared__ T lds[2 * (TY + 2) * (TX + 2)];
+    using C = typename CmpOf<T, A>::type;
+    __shared__ C lds[2 * (TY + 2) * (TX + 2)];
 
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
     if (tile >= L.ntiles) return;
@@ -623,9 +634,9 @@ __global__ __launch_bounds__(TX* TY) void k_stencil(Launch<T, A> L, const T* xin
     pc.sgn = vd.sgn;
     pc.np = dim_of(vd, vd.primary);
     pc.first = st->first;
-    pc.alpha = T(st->alpha);
-    pc.beta = T(st->beta);
-    pc.omega = T(st->omega);
+    pc.alpha = C(st->alpha);
+    pc.beta = C(st->beta);
+    pc.omega = C(st->omega);
     pc.code = vd.code;
     {
         const bool odd = (st->phase & 1) != 0;
@@ -646,7 +657,7 @@ __global__ __launch_bounds__(TX* TY) void k_stencil(Launch<T, A> L, const T* xin
     Acc acc[NPA];
 #pragma unroll
     for (int k = 0; k < NPA; ++k) acc[k].init();
-    march<T, Ops, PassCtx<T, A>, Acc, NPA, TX, TY>(pc, vd, tile - vd.tile_base, lds, acc);
+    march<C, Ops, PassCtx<T, A>, Acc, NPA, TX, TY>(pc, vd, tile - vd.tile_base, lds, acc);
     if constexpr (NP > 0) block_reduce_store<Acc, NP, TX * TY>(acc, L.part + (long long)tile * kPart);
 }
 
@@ -661,11 +672,12 @@ __global__ __launch_bounds__(TX* TY) void k_elem(Launch<T, A> L) {
     const VolDesc& vd = L.vol[v];
     const VolState<A>* st = L.st + v;
     if (KIND == E_MIXUPD) {
-        if (st->status != ST_DONE) return;
+        if (st->status != ST_UPDATE && st->status != ST_FRESH) return;
     } else if (st->status != ST_RUNNING) {
         return;
     }
-    const Coef<T> cf = L.coef[v];
+    using C = typename CmpOf<T, A>::type;
+    const Coef<C> cf = L.coef[v];
     const bool odd = (st->phase & 1) != 0;
     const T* Pn = (const T*)(odd ? L.vs.p[0] : L.vs.p[1]);   // p written by this iteration
     const T* Vn = (const T*)(odd ? L.vs.v[0] : L.vs.v[1]);
@@ -689,36 +701,40 @@ __global__ __launch_bounds__(TX* TY) void k_elem(Launch<T, A> L) {
             const long long lin = (long long)x + (long long)nx * y + plane * z;
             const long long i = vd.off + lin;
             if (KIND == E_MIXUPD) {
-                L.vs.x64[i] = L.vs.x64[i] + (double)((const float*)L.vs.x)[i];
+                float* D = (float*)L.vs.x;
+                if (st->refinements > 0 && !st->discard) L.vs.x64[i] = L.vs.x64[i] + (double)D[i];
+                D[i] = 0.0f;
                 continue;
             }
             const int code = vd.code[lin];
             const int bx = box_flags(x, y, z, nx, ny, nz);
-            const T inv = T(1) / diag_of<T>(code, bx, primary, cf);
+            const C inv = C(1) / diag_of<C>(code, bx, primary, cf);
             T* X = (T*)L.vs.x;
             T* R = (T*)L.vs.r;
             if (KIND == E_BICG3) {
-                const T p = Pn[i];
-                const T vv = Vn[i];
-                const T t = ((const T*)L.vs.t)[i];
-                const T r0 = ((const T*)L.vs.r0)[i];
-                const T r = R[i];
-                const T yv = inv * p;
-                const T s = r - T(alpha) * vv;
-                const T zv = inv * s;
-                X[i] = X[i] + (T(alpha) * yv + T(omega) * zv);
-                const T rn = s - T(omega) * t;
-                R[i] = rn;
-                acc[0].add(rn, rn);
-                acc[1].add(r0, rn);
+                const C p = C(Pn[i]);
+                const C vv = C(Vn[i]);
+                const C t = C(((const T*)L.vs.t)[i]);
+                const C r0 = C(((const T*)L.vs.r0)[i]);
+                const C r = C(R[i]);
+                const C yv = inv * p;
+                const C s = r - C(alpha) * vv;
+                const C zv = inv * s;
+                X[i] = T(C(X[i]) + (C(alpha) * yv + C(omega) * zv));
+                const C rn = s - C(omega) * t;
+                const T rs = T(rn);   // the stored residual is what the next iteration sees
+                R[i] = rs;
+                acc[0].add(C(rs), C(rs));
+                acc[1].add(r0, C(rs));
             } else {   // E_CG2
-                const T p = Pn[i];
-                const T q = ((const T*)L.vs.t)[i];
-                X[i] = X[i] + T(alpha) * p;
-                const T rn = R[i] - T(alpha) * q;
-                R[i] = rn;
-                acc[0].add(rn, rn);
-                acc[1].add(rn, inv * rn);
+                const C p = C(Pn[i]);
+                const C q = C(((const T*)L.vs.t)[i]);
+                X[i] = T(C(X[i]) + C(alpha) * p);
+                const C rn = C(R[i]) - C(alpha) * q;
+                const T rs = T(rn);
+                R[i] = rs;
+                acc[0].add(C(rs), C(rs));
+                acc[1].add(C(rs), inv * C(rs));
             }
         }
     }
@@ -747,6 +763,60 @@ __device__ __forceinline__ void bicg_loop_head(VolState<A>* s, A rr0) {
         return;
     }
     s->beta = (s->rho / s->rho_old) * (s->alpha / s->omega);
+    s->status = ST_RUNNING;
+}
+
+// Mixed form, end of an iteration.  Reliable update (x64 += d, residual replaced by the FP64
+// true residual) when the recursive residual has dropped by delta since the last one, when it
+// meets the target (the true residual decides) or at the iteration cap.  With FP32-stored
+// vectors r0.r loses its accuracy once r0 and r are nearly orthogonal, so below
+// restart_cos the update also resets the shadow residual r0 = r (Eigen's restart rule,
+// which keeps p and v).  A blow-up (residual up 100x since the last update or non-finite)
+// drops the correction and starts afresh from x64.
+template <class A>
+__device__ __forceinline__ void mixed_after_iteration(VolState<A>* s, A rr0) {
+    if (!isfinite(s->rsq) || s->rsq > 1e4 * s->rsq_ref) {
+        s->status = ST_FRESH;
+        s->discard = 1;
+        return;
+    }
+    if (!(s->rsq > s->tol2) || s->rsq < s->delta2 * s->rsq_ref || s->iter >= s->maxiter) {
+        s->status = ST_UPDATE;
+        return;
+    }
+    if (!isfinite(rr0) || fabs(rr0) < s->eps2 * s->r0sq || fabs(rr0) < s->restart_cos * sqrt(s->r0sq * s->rsq)) {
+        s->status = ST_UPDATE;
+        s->reshadow = 1;
+        return;
+    }
+    if (s->omega == A(0)) {
+        s->status = ST_FRESH;
+        return;
+    }
+    s->rho_old = s->rho;
+    s->rho = rr0;
+    s->beta = (s->rho / s->rho_old) * (s->alpha / s->omega);
+    s->status = ST_RUNNING;
+}
+
+// Mixed form, CG: the same update / fresh-start rules; between updates the usual PCG
+// recurrence (beta = r.z / previous r.z).
+template <class A>
+__device__ __forceinline__ void mixed_cg_after_iteration(VolState<A>* s, A rz) {
+    s->iter += 1;
+    if (!isfinite(s->rsq) || s->rsq > 1e4 * s->rsq_ref) {
+        s->status = ST_FRESH;
+        s->discard = 1;
+        return;
+    }
+    if (!(s->rsq > s->tol2) || s->rsq < s->delta2 * s->rsq_ref || s->iter >= s->maxiter) {
+        s->status = ST_UPDATE;
+        return;
+    }
+    const A absold = s->absnew;
+    s->absnew = rz;
+    s->beta = s->absnew / absold;
+    s->first = 0;
     s->status = ST_RUNNING;
 }
 
@@ -810,17 +880,22 @@ __global__ __launch_bounds__(64) void k_fin(Launch<T, A> L) {
         }
         case F_BICG1:
             s->alpha = s->rho / tot[0];
+            if (s->mode == 1 && !isfinite(s->alpha)) s->status = ST_FRESH;   // breakdown: start afresh
             break;
         case F_BICG2: {
             const A tmp = tot[1];
             s->omega = (tmp > A(0)) ? tot[0] / tmp : A(0);
+            if (s->mode == 1 && !isfinite(s->omega)) s->status = ST_FRESH;
             break;
         }
         case F_BICG3: {
             s->rsq = tot[0];
             s->iter += 1;
             s->phase += 1;
-            bicg_loop_head(s, tot[1]);
+            if (s->mode == 1)
+                mixed_after_iteration(s, tot[1]);
+            else
+                bicg_loop_head(s, tot[1]);
             break;
         }
         case F_INIT_CG: {
@@ -845,10 +920,15 @@ __global__ __launch_bounds__(64) void k_fin(Launch<T, A> L) {
         }
         case F_CG1:
             s->alpha = s->absnew / tot[0];
+            if (s->mode == 1 && !isfinite(s->alpha)) s->status = ST_FRESH;
             break;
         case F_CG2: {
             s->rsq = tot[0];
             s->phase += 1;
+            if (s->mode == 1) {
+                mixed_cg_after_iteration(s, tot[1]);
+                return;
+            }
             if (s->rsq < s->thresh) {
                 s->status = ST_DONE;
                 return;
@@ -1018,49 +1098,55 @@ namespace {
 
 struct MixCtx {
     Coef<double> c;
-    int primary, sgn, np, first;
+    int primary, sgn, np, first, fresh, reshadow, solver;
     const uint8_t* code;
     double* x64;
     float* r;
     float* r0;
-    float* d;
     float* p_cur;
     float* v_cur;
 };
 
-// FP64 residual of the refinement: u = x64 (x0 on the first step); r64 = b - A x64 and the
-// FP32 inner system starts from d = 0 with r = r0 = float(r64), p = v = 0.
+// FP64 true residual of the mixed form: u = x64 (x0 on the first step); r64 = b - A x64
+// replaces the FP32 residual; a fresh start also resets r0 = r and p = v = 0.
 struct MixOps {
-    static constexpr int NP = 3;   // |r64|^2, |b|^2, |float(r64)|^2
-    template <class C>
-    __device__ static Raw<double> fetch(const C& pc, long long idx, long long lin, bool centre) {
+    static constexpr int NP = 4;   // |r64|^2, |b|^2, |float(r64)|^2, r0.float(r64)
+    template <class X>
+    __device__ static Raw<double> fetch(const X& pc, long long idx, long long lin, bool centre) {
         Raw<double> w;
         if (!pc.first) w.a = pc.x64[idx];
+        if (centre && !pc.reshadow) w.e = (double)pc.r0[idx];
         w.code = pc.code[lin];
         return w;
     }
-    template <class C>
-    __device__ static Pt<double> compute(const C& pc, const Raw<double>& w, int bx, int k) {
+    template <class X>
+    __device__ static Pt<double> compute(const X& pc, const Raw<double>& w, int bx, int k) {
         Pt<double> p;
         p.code = w.code;
         p.u = pc.first ? x0_of<double>(k, pc.np, pc.sgn) : w.a;
         p.dg = diag_of<double>(w.code, bx, pc.primary, pc.c);
+        p.e = w.e;
         return p;
     }
-    template <class C, class Acc>
-    __device__ static void epi(const C& pc, long long idx, int k, double Au, const Pt<double>& pt, Acc (&acc)[NP]) {
+    template <class X, class Acc>
+    __device__ static void epi(const X& pc, long long idx, int k, double Au, const Pt<double>& pt, Acc (&acc)[NP]) {
         const double b = rhs_of<double>(pt.code, k, pc.np, pc.c);
         const double r64 = b - Au;
         const float r32 = (float)r64;
         if (pc.first) pc.x64[idx] = pt.u;
         pc.r[idx] = r32;
-        pc.r0[idx] = r32;
-        pc.d[idx] = 0.0f;
-        pc.p_cur[idx] = 0.0f;
-        pc.v_cur[idx] = 0.0f;
+        if (pc.reshadow) pc.r0[idx] = r32;
+        if (pc.fresh) {
+            pc.p_cur[idx] = 0.0f;
+            pc.v_cur[idx] = 0.0f;
+        }
         acc[0].add(r64, r64);
         acc[1].add(b, b);
         acc[2].add(r32, r32);
+        if (pc.solver == 1)
+            acc[3].add((double)r32, (1.0 / pt.dg) * (double)r32);   // CG: r.(M^-1 r)
+        else
+            acc[3].add(pc.reshadow ? (double)r32 : pt.e, (double)r32);   // BiCGSTAB: r0.r
     }
 };
 
@@ -1072,80 +1158,105 @@ __global__ __launch_bounds__(TX* TY) void k_mixres(Launch<float, double> L, cons
     const int v = L.tile_vol[tile];
     const VolDesc& vd = L.vol[v];
     const VolState<double>* st = L.st + v;
-    if (st->status != ST_DONE) return;   // volumes whose inner cycle ended take a refinement step
+    if (st->status != ST_UPDATE && st->status != ST_FRESH) return;
     MixCtx pc;
     pc.c = coef64[v];
     pc.primary = vd.primary;
     pc.sgn = vd.sgn;
     pc.np = dim_of(vd, vd.primary);
     pc.first = (st->refinements == 0);
+    pc.fresh = (st->status == ST_FRESH);
+    pc.solver = st->solver;
+    pc.reshadow = (pc.fresh || st->reshadow) && pc.solver == 0;
     pc.code = vd.code;
     const bool odd = (st->phase & 1) != 0;
     pc.x64 = L.vs.x64;
     pc.r = (float*)L.vs.r;
     pc.r0 = (float*)L.vs.r0;
-    pc.d = (float*)L.vs.x;
     pc.p_cur = (float*)(odd ? L.vs.p[1] : L.vs.p[0]);
     pc.v_cur = (float*)(odd ? L.vs.v[1] : L.vs.v[0]);
-    AccD acc[3];
+    AccD acc[4];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) acc[k].init();
-    march<double, MixOps, MixCtx, AccD, 3, TX, TY>(pc, vd, tile - vd.tile_base, lds, acc);
-    block_reduce_store<AccD, 3, TX * TY>(acc, L.part + (long long)tile * kPart);
+    for (int k = 0; k < 4; ++k) acc[k].init();
+    march<double, MixOps, MixCtx, AccD, 4, TX, TY>(pc, vd, tile - vd.tile_base, lds, acc);
+    block_reduce_store<AccD, 4, TX * TY>(acc, L.part + (long long)tile * kPart);
 }
 
-// Refinement step finalize: record the FP64 true residual; if it meets the outer target the
-// volume is done, else start an inner BiCGSTAB on d with r = r0 = float(r64).
+// Finalize of a reliable update / fresh start: the FP64 test decides whether the volume is
+// finished; otherwise BiCGSTAB continues on the replaced residual (a fresh start takes the
+// Eigen initial state with r0 = r).
 __global__ __launch_bounds__(64) void k_fin_mixres(Launch<float, double> L) {
     const int v = blockIdx.x;
     if (v >= L.nvol) return;
     VolState<double>* s = L.st + v;
-    if (s->status != ST_DONE) return;
+    const int status = s->status;
+    if (status != ST_UPDATE && status != ST_FRESH) return;
     const VolDesc& vd = L.vol[v];
     const int lane = threadIdx.x;
-    AccD sums[3];
-    for (int k = 0; k < 3; ++k) sums[k].init();
+    AccD sums[4];
+    for (int k = 0; k < 4; ++k) sums[k].init();
     for (int t = lane; t < vd.ntiles; t += 64) {
         const double* p = L.part + (long long)(vd.tile_base + t) * kPart;
-        for (int k = 0; k < 3; ++k) sums[k].add2(p[2 * k], p[2 * k + 1]);
+        for (int k = 0; k < 4; ++k) sums[k].add2(p[2 * k], p[2 * k + 1]);
     }
-    double tot[3];
-    for (int k = 0; k < 3; ++k) {
+    double tot[4];
+    for (int k = 0; k < 4; ++k) {
         wave_combine(sums[k]);
         tot[k] = sums[k].hi + sums[k].lo;
     }
     if (lane != 0) return;
-    const bool first = (s->refinements == 0);
+    if (s->refinements == 0) {
+        s->b64sq = tot[1];
+        s->tol2 = s->otol * s->otol * s->b64sq;
+    }
+    s->refinements += 1;
+    s->discard = 0;
     s->r64sq = tot[0];
-    if (first) s->b64sq = tot[1];
-    if (!first) s->total_iter += s->iter;
-    // outer test on the FP64 true residual
-    if (s->b64sq == 0.0 || !(s->r64sq > s->otol * s->otol * s->b64sq) || s->refinements >= s->max_refine) {
+    if (s->b64sq == 0.0 || !(s->r64sq > s->otol * s->otol * s->b64sq) || s->iter >= s->maxiter) {
         s->status = ST_IDLE;
         return;
     }
-    s->refinements += 1;
-    // inner tolerance: ask the FP32 cycle for the reduction still needed (half the remaining
-    // gap to the outer target), never below the floor and never weaker than 1/2
-    double tin = 0.5 * s->otol * sqrt(s->b64sq / s->r64sq);
-    tin = tin < s->tol_floor ? s->tol_floor : tin;
-    tin = tin > 0.5 ? 0.5 : tin;
-    s->tol = tin;
-    // inner system set-up (Eigen BiCGSTAB initial state with x = 0, r0 = r)
+    const int reshadow = s->reshadow;
+    s->reshadow = 0;
     s->rsq = tot[2];
-    s->r0sq = tot[2];
-    s->bsq = tot[2];
-    s->rho = 1.0;
-    s->alpha = 1.0;
-    s->omega = 1.0;
-    s->restarts = 0;
-    s->tol2 = s->tol * s->tol * s->bsq;
-    s->iter = 0;
-    if (s->bsq == 0.0) {
-        s->status = ST_DONE;
+    s->rsq_ref = tot[2];
+    if (s->solver == 1) {   // CG: continue with beta from the replaced residual, or p = z afresh
+        if (status == ST_FRESH) {
+            if (s->refinements > 1) s->restarts += 1;
+            s->first = 1;
+            s->absnew = tot[3];
+        } else {
+            const double absold = s->absnew;
+            s->absnew = tot[3];
+            s->beta = s->absnew / absold;
+            s->first = 0;
+        }
+        s->status = ST_RUNNING;
         return;
     }
-    bicg_loop_head(s, tot[2]);
+    if (status == ST_FRESH) {
+        if (s->refinements > 1) s->restarts += 1;
+        s->rho = 1.0;
+        s->alpha = 1.0;
+        s->omega = 1.0;
+    }
+    if (status == ST_FRESH || reshadow) s->r0sq = tot[2];
+    if (reshadow && status != ST_FRESH) s->restarts += 1;
+    const double rho_new = tot[3];
+    if (status != ST_FRESH && !reshadow &&
+        (!isfinite(rho_new) || fabs(rho_new) < s->restart_cos * sqrt(s->r0sq * s->rsq))) {
+        s->status = ST_UPDATE;   // the replaced residual is too orthogonal to r0: reset r0 next
+        s->reshadow = 1;
+        return;
+    }
+    if (s->omega == 0.0) {
+        s->status = ST_FRESH;
+        return;
+    }
+    s->rho_old = s->rho;
+    s->rho = rho_new;
+    s->beta = (s->rho / s->rho_old) * (s->alpha / s->omega);
+    s->status = ST_RUNNING;
 }
 
 }  // namespace

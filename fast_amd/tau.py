@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import ctypes as C
 import enum
+import weakref
 from dataclasses import dataclass, field
 from typing import Optional, Sequence
 
@@ -108,6 +109,7 @@ class Context:
 
     def __init__(self, device: int = 0, hbm_budget: int = 0):
         self._h = C.c_void_p()
+        self._volumes = weakref.WeakSet()
         check(lib().fast_ctx_create(device, hbm_budget, C.byref(self._h)), "fast_ctx_create")
 
     @property
@@ -115,6 +117,9 @@ class Context:
         return self._h
 
     def close(self):
+        # volumes first: their handles point into this context
+        for v in list(self._volumes):
+            v.close()
         if self._h:
             lib().fast_ctx_destroy(self._h)
             self._h = C.c_void_p()
@@ -152,6 +157,7 @@ class Volume:
         self._h = C.c_void_p()
         check(lib().fast_volume_create(ctx.handle, _u8(m), nx, ny, nz, C.byref(self._h)), "fast_volume_create",
               ctx.handle)
+        ctx._volumes.add(self)
 
     @property
     def handle(self):
@@ -163,7 +169,8 @@ class Volume:
 
     def close(self):
         if self._h:
-            lib().fast_volume_destroy(self._h)
+            if self.ctx.handle:
+                lib().fast_volume_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

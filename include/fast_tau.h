@@ -44,7 +44,9 @@ enum fast_dir { FAST_X_POS = 0, FAST_X_NEG = 1, FAST_Y_POS = 2, FAST_Y_NEG = 3, 
 /* DiffusionSolverType, VolumeMeasures.h:15-18 */
 enum fast_solver { FAST_BICGSTAB = 0, FAST_CG = 1 };
 /* Precision: FP64 and FP32 are the reference's `-p double|float` (TauModule.h:19,125-129);
- * MIXED is this library's FP32-stencil / FP64-reduction / FP64 iterative-refinement form. */
+ * MIXED is this library's form: BiCGSTAB with FP32 vectors and stencil, FP64 dot products
+ * and scalars, the solution accumulated in FP64 and the residual periodically replaced by
+ * the FP64 true residual (iterative refinement inside one Krylov process). */
 enum fast_precision { FAST_FP64 = 0, FAST_FP32 = 1, FAST_MIXED = 2 };
 /* Solve status (Solver<T>::Status, fastlib/include/fastlib/solver/Solver.h:47-51, extended) */
 enum fast_status { FAST_SOLVED = 0, FAST_NO_CONVERGENCE = 1, FAST_DIVERGED = 2, FAST_FAILED = 3 };
@@ -59,8 +61,7 @@ typedef struct fast_tau_params {
     double d_pore;        /* coeffs[0]; the reference CLI passes float 1.0f (TauModule.h:70) */
     double d_solid;       /* coeffs[1]; the reference CLI passes float 0.001f */
     double porosity;      /* < 0: computed on the device as float(#zero/N) (TauModule.h:62) */
-    double mixed_inner_tol; /* MIXED only: floor of the per-cycle FP32 relative residual reduction (0 = 1e-6);
-                               each cycle asks for half the remaining gap to the outer target */
+    double mixed_inner_tol; /* MIXED only: residual reduction between FP64 reliable updates (0 = 0.1) */
     double mixed_outer_tol; /* MIXED only: FP64 true relative residual target (0 = `tolerance`) */
     int verbose;
     int reserved[3];
@@ -73,10 +74,10 @@ typedef struct fast_tau_result {
     double dc;            /* mean boundary flux (VolumeMeasures.cpp:274) */
     double rel_residual;  /* solver error: sqrt(|r|^2/|b|^2) of the recursive residual */
     double true_rel_residual; /* MIXED: FP64 true residual at exit; otherwise -1 */
-    long iterations;      /* Krylov iterations (MIXED: total inner iterations) */
+    long iterations;      /* Krylov iterations */
     int status;           /* fast_status */
-    int restarts;         /* BiCGSTAB restarts (rho breakdown) */
-    int refinements;      /* MIXED: outer refinement cycles */
+    int restarts;         /* BiCGSTAB restarts (rho breakdown; MIXED: fresh starts) */
+    int refinements;      /* MIXED: FP64 true-residual replacements (reliable updates) */
     int reserved;
     double solve_ms;      /* device time of the solve (events on the solve stream) */
 } fast_tau_result;

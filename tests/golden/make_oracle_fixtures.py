@@ -5,7 +5,8 @@ minutes per solve at 256^3, too slow to run inside the test suite, so its result
 committed here and the GPU parity tests compare against them.  Each record stores the
 SHA-256 of the generated mask so a test can prove it solved the same volume.
 
-  python tests/golden/make_oracle_fixtures.py config2   # 256^3 ellipsoids, CG + BiCGSTAB, X/Y/Z
+  python tests/golden/make_oracle_fixtures.py config2             # 256^3 ellipsoids, CG + BiCGSTAB, X/Y/Z
+  python tests/golden/make_oracle_fixtures.py config2_converged   # same volume, CG to 1e-11
 """
 from __future__ import annotations
 
@@ -53,3 +54,7 @@ if __name__ == "__main__":
     which = sys.argv[1] if len(sys.argv) > 1 else "config2"
     if which == "config2":
         run("config2", config2_mask(), [(0, 0), (1, 0), (0, 2), (1, 2), (0, 4), (1, 4)])
+    elif which == "config2_converged":
+        # the converged tau (CG to 1e-11): the yardstick for tau accuracy at 256^3, where
+        # BiCGSTAB stopped at 1e-6 is still up to ~1e-4 away from it
+        run("config2_converged", config2_mask(), [(1, 0), (1, 2), (1, 4)], tol=1e-11)

@@ -49,7 +49,7 @@ def test_batch_equals_solo_bit_exact(ctx):
         m = F.generate_ellipsoids((nx, ny, nz), 6.0, float(np.exp(rng.uniform(np.log(0.2), np.log(5)))), 0.55, 100 + i)
         vols.append(ctx.volume(m))
         params.append((i % 3) * 2)
-    for prec, solver in (("double", 0), ("double", 1), ("mixed", 0)):
+    for prec, solver in (("double", 0), ("double", 1), ("mixed", 0), ("mixed", 1)):
         ps = [T.make_params(d, solver, prec, 1e-6, 10000) for d in params]
         batch = T.solve_batch(ctx, vols, ps)
         for v, p, b in zip(vols, ps, batch):
