@@ -2,23 +2,24 @@
 
 Workload (the headline configuration): one 256^3 voxel volume of overlapping ellipsoids,
 semi-axes (12, 4.2, 4.2) (a = 12, r = 0.35), uniformly random orientations and centres,
-filled to solid fraction 0.65 (seed 0).  tau is solved in the x, y and z directions with
-BiCGSTAB at tolerance 1e-6.
+filled to solid fraction 0.65 (seed 0); tau in the x, y and z directions at tolerance 1e-6.
 
-  * plain FP64 GPU solve (the drop-in): one direction per solve, timed per solve;
-  * mixed + batched (the headline form): FP32 stencil/vectors, FP64 reductions and FP64
-    iterative refinement, the three directions solved as one batched solve.
-A "step" is one batched mixed solve of the three directions.  value = mixed-batched
-solves/s.  Inputs are resident in HBM (fast_volume) before the timed region.
+  * plain FP64 GPU solve (the drop-in): BiCGSTAB and CG, one direction per solve;
+  * mixed + batched (the headline form): CG with FP32 vectors in HBM, FP64 arithmetic and
+    dot products, the solution accumulated in FP64 with reliable updates of the residual;
+    the three directions solved as one batched solve.
+A "step" is one batched mixed solve of the three directions; value = mixed-batched solves/s.
+Inputs are resident in HBM (fast_volume) before the timed region.
 
-Also reported: the FP64 solve rate, tau errors of the mixed form against the FP64 GPU
-solve and against the committed CPU-oracle results, the roofline of the dominant kernel
-(BiCGSTAB fused pass 1) timed with HIP events on the solve stream, and the CPU oracle
-timed on a bounded sample on this host.
+tau accuracy is stated against the converged FP64 tau (CG to 1e-11): at 256^3 the FP64
+BiCGSTAB tau at tolerance 1e-6 is itself up to ~1e-4 away from it.  Also reported: the
+roofline of the dominant kernel (timed with HIP events on the solve stream; the rocprofv3
+summary under profiles/ agrees) and the CPU oracle timed on a bounded sample on this host.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -33,36 +34,46 @@ sys.path.insert(0, str(ROOT))
 METRIC = "tau solves/s & ms/solve @256^3 (FP64 vs mixed+batched); p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
 DIRS = (0, 2, 4)          # X_POS, Y_POS, Z_POS
+DIRNAMES = ["X_POS", "X_NEG", "Y_POS", "Y_NEG", "Z_POS", "Z_NEG"]
 TOL = 1e-6
+BICG, CG = 0, 1
 
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def pass1_bytes_per_voxel(vec_bytes: int) -> int:
-    # BiCGSTAB fused pass 1 reads r, p_old, v_old, r0 and the stencil code byte, writes p, v
-    return 6 * vec_bytes + 1
+def converged_taus(vol, T) -> tuple:
+    fx = ROOT / "tests" / "golden" / "config2_converged_oracle.json"
+    out, src = {}, "GPU FP64 CG tol 1e-11 (bit-identical to the CPU oracle's recurrences)"
+    if fx.exists():
+        recs = json.loads(fx.read_text())
+        if all(f"CG_{DIRNAMES[d]}" in recs for d in DIRS):
+            return {d: recs[f"CG_{DIRNAMES[d]}"]["tau"] for d in DIRS}, "CPU oracle CG tol 1e-11"
+    for d in DIRS:
+        res, _ = vol.solve(T.make_params(d, CG, "double", 1e-11, 20000))
+        out[d] = res.tau
+    return out, src
 
 
 def cpu_baseline(mask: np.ndarray, gpu_iters: float, threads: int, sample_iters: int) -> dict:
-    """Time the CPU oracle (restated reference `--cpu` BiCGSTAB, FP64) on a bounded sample:
-    assembly alone, then assembly + `sample_iters` iterations; a full solve is extrapolated
-    with the FP64 GPU iteration count (same algorithm)."""
+    """Time the CPU oracle (the restated reference `--cpu` CG, FP64, the CLI's default solver)
+    on a bounded sample: assembly alone, then assembly + `sample_iters` iterations; a full
+    solve is extrapolated with the FP64 GPU CG iteration count (same recurrences)."""
     from oracle import oracle as O
     t = time.perf_counter()
-    O.tau(mask, 0, solver=0, tol=TOL, max_iter=0, threads=threads)
+    O.tau(mask, 0, solver=CG, tol=TOL, max_iter=0, threads=threads)
     t_setup = time.perf_counter() - t
     t = time.perf_counter()
-    O.tau(mask, 0, solver=0, tol=TOL, max_iter=sample_iters, threads=threads)
+    O.tau(mask, 0, solver=CG, tol=TOL, max_iter=sample_iters, threads=threads)
     t_run = time.perf_counter() - t
     per_iter = max(t_run - t_setup, 1e-9) / sample_iters
     solve_s = t_setup + per_iter * gpu_iters
     return {
         "value": 1.0 / solve_s, "unit": "solves/s", "cores": threads, "kind": "port",
-        "sample": (f"256^3 config volume, X_POS BiCGSTAB FP64: assembly ({t_setup:.2f} s) + {sample_iters} "
-                   f"iterations ({per_iter * 1e3:.0f} ms/iteration) timed, extrapolated to {gpu_iters:.0f} "
-                   f"iterations = {solve_s:.1f} s/solve"),
+        "sample": (f"256^3 volume, X_POS, CG FP64 (oracle restating the reference --cpu path): assembly "
+                   f"{t_setup:.2f} s + {sample_iters} iterations at {per_iter * 1e3:.0f} ms each, extrapolated to "
+                   f"{gpu_iters:.0f} iterations = {solve_s:.1f} s per solve"),
     }
 
 
@@ -73,11 +84,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--cpu-iters", type=int, default=12)
+    ap.add_argument("--cpu-iters", type=int, default=25)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(independent volumes shard one queue per GPU with no communication)", file=sys.stderr)
+              "(independent volumes shard as one queue per GPU with no communication)", file=sys.stderr)
         return 2
 
     import fast_amd as F
@@ -91,20 +102,25 @@ def main() -> int:
     vol = ctx.volume(mask)
     nvox = mask.size
 
-    # ---- plain FP64 drop-in, one direction per solve ----
-    fp64 = {}
-    log("[bench] FP64 BiCGSTAB warm-up")
-    vol.solve(T.make_params(0, 0, "double", TOL, 10000))
-    for d in DIRS:
-        t = time.perf_counter()
-        res, _ = vol.solve(T.make_params(d, 0, "double", TOL, 10000))
-        fp64[d] = (res, time.perf_counter() - t)
-        log(f"[bench] FP64 d={d} tau={res.tau:.9f} it={res.iterations} {fp64[d][1] * 1e3:.1f} ms")
-    fp64_ms = 1e3 * float(np.mean([fp64[d][1] for d in DIRS]))
+    conv, conv_src = converged_taus(vol, T)
 
-    # ---- mixed + batched: the three directions in one batched solve ----
-    params = [T.make_params(d, 0, "mixed", TOL, 10000) for d in DIRS]
-    vols = [vol, vol, vol]
+    # ---- plain FP64 drop-in, one direction per solve, both solvers ----
+    log("[bench] FP64 warm-up")
+    vol.solve(T.make_params(0, BICG, "double", TOL, 10000))
+    fp64 = {}
+    for solver in (BICG, CG):
+        for d in DIRS:
+            t = time.perf_counter()
+            res, _ = vol.solve(T.make_params(d, solver, "double", TOL, 10000))
+            fp64[(solver, d)] = (res, time.perf_counter() - t)
+            log(f"[bench] FP64 solver={solver} d={d} tau={res.tau:.9f} it={res.iterations} "
+                f"{fp64[(solver, d)][1] * 1e3:.1f} ms err={abs(res.tau - conv[d]) / conv[d]:.1e}")
+    fp64_ms = {s: 1e3 * float(np.mean([fp64[(s, d)][1] for d in DIRS])) for s in (BICG, CG)}
+    fp64_rate = {s: 1e3 / fp64_ms[s] for s in (BICG, CG)}
+
+    # ---- mixed + batched: the three directions in one batched mixed CG solve ----
+    params = [T.make_params(d, CG, "mixed", TOL, 10000) for d in DIRS]
+    vols = [vol] * len(DIRS)
     for _ in range(args.warmup):
         T.solve_batch(ctx, vols, params)
     log(f"[bench] timing {args.steps} mixed batched steps")
@@ -114,38 +130,36 @@ def main() -> int:
     elapsed = time.perf_counter() - t0
     ms_per_step = 1e3 * elapsed / args.steps
     value = len(DIRS) * args.steps / elapsed
-    fp64_rate = 1e3 / fp64_ms
+    err_mixed = {d: abs(o.tau - conv[d]) / conv[d] for o, d in zip(out, DIRS)}
 
-    errs_gpu = [abs(o.tau - fp64[d][0].tau) / fp64[d][0].tau for o, d in zip(out, DIRS)]
-    fixture = ROOT / "tests" / "golden" / "config2_oracle.json"
-    errs_oracle = None
-    if n == 256 and fixture.exists():
-        fx = json.loads(fixture.read_text())
-        errs_oracle = {}
-        for o, d in zip(out, DIRS):
-            key = f"BICGSTAB_{['X_POS', 'X_NEG', 'Y_POS', 'Y_NEG', 'Z_POS', 'Z_NEG'][d]}"
-            if key in fx:
-                errs_oracle[key] = abs(o.tau - fx[key]["tau"]) / fx[key]["tau"]
-                errs_oracle["fp64_" + key] = abs(fp64[d][0].tau - fx[key]["tau"]) / fx[key]["tau"]
-
-    # ---- roofline of the dominant kernel (fused pass 1), HIP events on the solve stream ----
-    import ctypes as C
+    # ---- roofline of the dominant kernel, HIP events on the solve stream ----
     ms = (C.c_double * 4)()
-    lib().fast_debug_time_passes(vol.handle, 0, 2, 20, ms)
+    lib().fast_debug_time_passes(vol.handle, 0, 2, CG, 20, ms)
     mixed_pass = list(ms)
-    lib().fast_debug_time_passes(vol.handle, 0, 0, 20, ms)
-    fp64_pass = list(ms)
-    b32 = pass1_bytes_per_voxel(4) * nvox
-    b64 = pass1_bytes_per_voxel(8) * nvox
-    ach32 = b32 / (mixed_pass[0] * 1e-3) / 1e9
-    ach64 = b64 / (fp64_pass[0] * 1e-3) / 1e9
+    lib().fast_debug_time_passes(vol.handle, 0, 0, CG, 20, ms)
+    fp64_cg_pass = list(ms)
+    lib().fast_debug_time_passes(vol.handle, 0, 0, BICG, 20, ms)
+    fp64_bicg_pass = list(ms)
+    # mixed CG: stencil pass reads r, p_old and the code byte, writes p, q (4 FP32 vectors);
+    # update pass reads x, p, r, q and the code byte, writes x, r (6 FP32 vectors)
+    kernels = {
+        "k_stencil<float,double,64,8,CG1> (PCG stencil pass: p = M^-1 r + beta p, q = A p, p.q)":
+            (mixed_pass[0], (4 * 4 + 1) * nvox),
+        "k_elem<float,double,64,8,CG2> (PCG update pass: x, r, r.r, r.M^-1 r)":
+            (mixed_pass[1], (6 * 4 + 1) * nvox),
+    }
+    name, (kms, kbytes) = max(kernels.items(), key=lambda kv: kv[1][0])
+    ach = kbytes / (kms * 1e-3) / 1e9
+    others = {k: {"avg_launch_ms": v[0], "bytes_per_launch": v[1], "achieved_GBs": v[1] / (v[0] * 1e-3) / 1e9}
+              for k, v in kernels.items()}
 
     cpu = None
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         log(f"[bench] CPU oracle sample on {threads} threads")
-        cpu = cpu_baseline(mask, float(fp64[0][0].iterations), threads, args.cpu_iters)
+        cpu = cpu_baseline(mask, float(fp64[(CG, 0)][0].iterations), threads, args.cpu_iters)
 
+    best_fp64 = max(fp64_rate.values())
     line = {
         "metric": METRIC,
         "value": value,
@@ -157,35 +171,36 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 vectors/stencil + f64 reductions and refinement",
+        "dtype": "f32 storage, f64 arithmetic/reductions/solution",
         "data": "synthetic",
         "config": {
-            "workload": f"{n}^3 ellipsoids a=12 r=0.35 solid 0.65 seed 0; BiCGSTAB tol 1e-6; "
-                        "directions x,y,z as one batched mixed-precision solve per step",
+            "workload": f"{n}^3 ellipsoids a=12 r=0.35 solid 0.65 seed 0; tol 1e-6; x,y,z solved as one "
+                        "batched mixed-precision CG solve per step",
             "volume": [n, n, n], "solves_per_step": len(DIRS), "parallelism": "single GPU",
         },
-        "fp64_solves_per_s": fp64_rate,
-        "fp64_ms_per_solve": fp64_ms,
         "mixed_batched_ms_per_solve": ms_per_step / len(DIRS),
-        "speedup_mixed_batched_vs_fp64": value / fp64_rate,
-        "tau_fp64": {str(d): fp64[d][0].tau for d in DIRS},
-        "iters_fp64": {str(d): int(fp64[d][0].iterations) for d in DIRS},
-        "tau_mixed": {str(d): o.tau for o, d in zip(out, DIRS)},
-        "iters_mixed": {str(d): int(o.iterations) for o, d in zip(out, DIRS)},
-        "refinements_mixed": {str(d): int(o.refinements) for o, d in zip(out, DIRS)},
-        "tau_rel_err_mixed_vs_fp64_gpu": max(errs_gpu),
-        "tau_rel_err_vs_cpu_oracle": errs_oracle,
+        "fp64_bicgstab_solves_per_s": fp64_rate[BICG],
+        "fp64_cg_solves_per_s": fp64_rate[CG],
+        "speedup_vs_fastest_fp64": value / best_fp64,
+        "speedup_vs_fp64_cg": value / fp64_rate[CG],
+        "converged_tau": {DIRNAMES[d]: conv[d] for d in DIRS},
+        "converged_tau_source": conv_src,
+        "tau_rel_err_vs_converged": {
+            "mixed_batched_cg": {DIRNAMES[d]: err_mixed[d] for d in DIRS},
+            "fp64_bicgstab": {DIRNAMES[d]: abs(fp64[(BICG, d)][0].tau - conv[d]) / conv[d] for d in DIRS},
+            "fp64_cg": {DIRNAMES[d]: abs(fp64[(CG, d)][0].tau - conv[d]) / conv[d] for d in DIRS},
+        },
+        "iterations": {
+            "mixed_batched_cg": {DIRNAMES[d]: int(o.iterations) for o, d in zip(out, DIRS)},
+            "fp64_bicgstab": {DIRNAMES[d]: int(fp64[(BICG, d)][0].iterations) for d in DIRS},
+            "fp64_cg": {DIRNAMES[d]: int(fp64[(CG, d)][0].iterations) for d in DIRS},
+        },
         "roofline": {
-            "bound": "hbm", "kernel": "k_stencil<float,double,...,BICG1> (fused BiCGSTAB pass 1, mixed inner)",
-            "achieved": ach32, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach32 / HBM_PEAK_GBS,
-            "traffic": None, "bytes_per_launch": b32, "avg_launch_ms": mixed_pass[0],
+            "bound": "hbm", "kernel": name, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": ach / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": kbytes, "avg_launch_ms": kms,
         },
-        "roofline_fp64_pass1": {
-            "achieved": ach64, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach64 / HBM_PEAK_GBS,
-            "bytes_per_launch": b64, "avg_launch_ms": fp64_pass[0],
-        },
-        "pass_ms_mixed": mixed_pass,
-        "pass_ms_fp64": fp64_pass,
+        "kernels_mixed_cg": others,
+        "pass_ms": {"mixed_cg": mixed_pass, "fp64_cg": fp64_cg_pass, "fp64_bicgstab": fp64_bicg_pass},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

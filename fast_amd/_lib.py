@@ -73,7 +73,8 @@ class ServerConfig(C.Structure):
         ("hbm_budget", C.c_size_t),
         ("max_active", C.c_int),
         ("iters_per_round", C.c_int),
-        ("reserved", C.c_int * 4),
+        ("use_bicgstab", C.c_int),
+        ("reserved", C.c_int * 3),
     ]
 
 
@@ -89,6 +90,8 @@ PROTOTYPES = [
     ("fast_ctx_destroy", None, [_vp]),
     ("fast_ctx_last_error", C.c_char_p, [_vp]),
     ("fast_ctx_device_bytes", C.c_size_t, [_vp]),
+    ("fast_ctx_set_tile_shape", C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    ("fast_ctx_set_round", C.c_int, [_vp, C.c_int]),
     ("fast_volume_create", C.c_int, [_vp, _u8p, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
     ("fast_volume_create_device", C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
     ("fast_volume_destroy", None, [_vp]),
@@ -113,7 +116,7 @@ PROTOTYPES = [
     ("fast_debug_apply_f64", C.c_int, [_vp, C.c_int, _dp, _dp]),
     ("fast_debug_apply_f32", C.c_int, [_vp, C.c_int, _fp, _fp]),
     ("fast_debug_system_f64", C.c_int, [_vp, C.c_int, _dp, _dp, _dp]),
-    ("fast_debug_time_passes", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp]),
+    ("fast_debug_time_passes", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
 ]
 
 _lib = None

@@ -79,7 +79,7 @@ def build(verbose: bool = False) -> Path:
     lib = HERE / "libfasttau.so"
     tmp = HERE / "libfasttau.so.tmp"
     _run([str(ROCM / "bin" / "hipcc"), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp),
-          *[str(o) for o in objs], f"-L{ROCM / 'lib'}", "-lamdhip64", "-Wl,-soname,libfasttau.so"])
+          *[str(o) for o in objs], f"-L{ROCM / 'lib'}", "-lamdhip64", "-ldl", "-Wl,-soname,libfasttau.so"])
     os.replace(tmp, lib)
     cli_tmp = HERE / "fast.tmp"
     _run(["g++", *CXX_FLAGS, str(CSRC / "fast_cli.cpp"), "-o", str(cli_tmp), f"-L{HERE}", "-lfasttau",

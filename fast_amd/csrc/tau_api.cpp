@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <dlfcn.h>
 #include <cstdarg>
 #include <cstdlib>
 #include <cstdio>
@@ -159,8 +160,13 @@ template Coef<float> make_coef<float>(int, int, int, int, double, double);
 // leave lanes idle.  tz (planes per tile) is chosen from the volume alone so that a large
 // volume yields ~2048+ tiles.  Both depend only on the volume, which keeps per-volume
 // reductions independent of the batch.
+static bool shape_compiled(int tx, int ty) {
+    return (tx == 64 && (ty == 4 || ty == 8)) || (tx == 32 && (ty == 8 || ty == 16)) || (tx == 16 && ty == 16);
+}
+
 Shape choose_shape(int nx, int ny, int nz, int precision, const fast_ctx* ctx) {
     Shape s{64, 8, 16};
+    if (ctx->forced.tx) return ctx->forced;
     const auto key = shape_key(nx, ny, nz, precision);
     auto it = ctx->tuned.find(key);
     if (it != ctx->tuned.end()) return it->second;
@@ -175,18 +181,26 @@ Shape choose_shape(int nx, int ny, int nz, int precision, const fast_ctx* ctx) {
     return s;
 }
 
-// Tuned tile shapes: text lines "<size-class key> <tx> <ty> <tz>" written by the autotuner,
-// read from $FAST_TAU_TUNING or fast_amd/tuning_gfx950.txt next to the library.
+// Tuned tile shapes: text lines "<size-class key> <tx> <ty> <tz>" written by the autotuner
+// (fast_amd/autotune.py), read from $FAST_TAU_TUNING, else from tuning_gfx950.txt in the
+// directory of this library.
 void load_tuning(fast_ctx* ctx) {
-    const char* path = std::getenv("FAST_TAU_TUNING");
-    if (!path || !*path) return;
-    FILE* f = std::fopen(path, "r");
+    std::string path;
+    if (const char* e = std::getenv("FAST_TAU_TUNING")) path = e;
+    if (path.empty()) {
+        Dl_info info;
+        if (dladdr((void*)&load_tuning, &info) && info.dli_fname) {
+            path = info.dli_fname;
+            const size_t slash = path.find_last_of('/');
+            path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/tuning_gfx950.txt";
+        }
+    }
+    FILE* f = path.empty() ? nullptr : std::fopen(path.c_str(), "r");
     if (!f) return;
     char key[64];
     int tx, ty, tz;
     while (std::fscanf(f, "%63s %d %d %d", key, &tx, &ty, &tz) == 4) {
-        const bool known = (tx == 64 && (ty == 4 || ty == 8)) || (tx == 32 && (ty == 8 || ty == 16)) || (tx == 16 && ty == 16);
-        if (known && tz >= 1 && tz <= 256) ctx->tuned[key] = Shape{tx, ty, tz};
+        if (shape_compiled(tx, ty) && tz >= 1 && tz <= 256) ctx->tuned[key] = Shape{tx, ty, tz};
     }
     std::fclose(f);
 }
@@ -276,6 +290,23 @@ void fast_ctx_destroy(fast_ctx* ctx) {
 }
 
 const char* fast_ctx_last_error(const fast_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int fast_ctx_set_tile_shape(fast_ctx* ctx, int tx, int ty, int tz) {
+    if (!ctx) return FAST_E_BADARG;
+    if (tx == 0) {
+        ctx->forced = Shape{0, 0, 0};
+        return FAST_OK;
+    }
+    if (!shape_compiled(tx, ty) || tz < 1 || tz > 256) return FAST_E_BADARG;
+    ctx->forced = Shape{tx, ty, tz};
+    return FAST_OK;
+}
+
+int fast_ctx_set_round(fast_ctx* ctx, int iterations) {
+    if (!ctx || iterations < 0 || iterations > 1024) return FAST_E_BADARG;
+    ctx->chunk = iterations ? iterations : 16;
+    return FAST_OK;
+}
 
 size_t fast_ctx_device_bytes(const fast_ctx* ctx) { return ctx ? ctx->held : 0; }
 
@@ -368,7 +399,7 @@ struct Batch {
     bool mixed = false;
     std::vector<VolDesc> hvol;
     using C = typename CmpOf<T, A>::type;
-    std::vector<int> htile, hplane;
+    std::vector<int> htile, hplane, hplane_k;
     std::vector<Coef<C>> hcoef;
     std::vector<Coef<double>> hcoef64;
     std::vector<VolState<A>> hst;
@@ -380,6 +411,7 @@ struct Batch {
     Coef<double>* coef64 = nullptr;
     double* plane_out = nullptr;
     int* plane_vol = nullptr;
+    int* plane_k = nullptr;
     hipGraphExec_t gexec = nullptr;   // one captured polling round of this batch
 
     ~Batch() {
@@ -473,10 +505,14 @@ static int prepare(Batch<T, A>& B) {
     B.total = total;
     B.htile.resize(tiles);
     B.hplane.resize(planes);
+    B.hplane_k.resize(planes);
     for (int i = 0; i < nv; ++i) {
         for (int t = 0; t < B.hvol[i].ntiles; ++t) B.htile[B.hvol[i].tile_base + t] = i;
         const int np = B.hvol[i].primary == 0 ? B.hvol[i].nx : (B.hvol[i].primary == 1 ? B.hvol[i].ny : B.hvol[i].nz);
-        for (int k = 0; k < np; ++k) B.hplane[B.hvol[i].plane_base + k] = i;
+        for (int k = 0; k < np; ++k) {
+            B.hplane[B.hvol[i].plane_base + k] = i;
+            B.hplane_k[B.hvol[i].plane_base + k] = k;
+        }
     }
     // meta layout
     size_t o = 0;
@@ -494,6 +530,8 @@ static int prepare(Batch<T, A>& B) {
     o = align_up(o + sizeof(double) * 8 * (size_t)tiles, 256);
     const size_t o_pvol = o;
     o = align_up(o + sizeof(int) * planes, 256);
+    const size_t o_pk = o;
+    o = align_up(o + sizeof(int) * planes, 256);
     const size_t o_pout = o;
     o = align_up(o + sizeof(double) * planes, 256);
     const size_t meta_bytes = o;
@@ -509,6 +547,7 @@ static int prepare(Batch<T, A>& B) {
     std::memcpy(h.data() + o_st, B.hst.data(), sizeof(VolState<A>) * nv);
     std::memcpy(h.data() + o_tile, B.htile.data(), sizeof(int) * tiles);
     std::memcpy(h.data() + o_pvol, B.hplane.data(), sizeof(int) * planes);
+    std::memcpy(h.data() + o_pk, B.hplane_k.data(), sizeof(int) * planes);
     hipError_t e = hipMemcpyAsync(B.meta, h.data(), meta_bytes, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);   // h goes out of scope
     if (!hip_ok(ctx, e, "batch upload")) return FAST_E_HIP;
@@ -524,6 +563,7 @@ static int prepare(Batch<T, A>& B) {
     B.L.part = (double*)(m + o_part);
     B.coef64 = (Coef<double>*)(m + o_coef64);
     B.plane_vol = (int*)(m + o_pvol);
+    B.plane_k = (int*)(m + o_pk);
     B.plane_out = (double*)(m + o_pout);
     T* vb = (T*)B.vecs;
     VecSet& vs = B.L.vs;
@@ -609,7 +649,8 @@ template <class T, class A, class TC>
 static int finish(Batch<T, A>& B, const TC* conc, const Coef<TC>* coef_dev, const std::vector<bool>& ok) {
     fast_ctx* ctx = B.ctx;
     const int nv = (int)B.jobs.size();
-    hipError_t e = ft::flux_planes<TC>(B.nplanes, B.L.vol, B.plane_vol, coef_dev, conc, B.plane_out, ctx->stream);
+    hipError_t e = ft::flux_planes<TC>(B.nplanes, B.L.vol, B.plane_vol, B.plane_k, coef_dev, conc, B.plane_out,
+                                       ctx->stream);
     std::vector<double> planes(B.nplanes);
     if (e == hipSuccess)
         e = hipMemcpyAsync(planes.data(), B.plane_out, sizeof(double) * B.nplanes, hipMemcpyDeviceToHost, ctx->stream);
@@ -762,12 +803,17 @@ int solve_jobs(fast_ctx* ctx, std::vector<Job>& jobs) {
         std::memset(j.res, 0, sizeof(*j.res));
         j.res->true_rel_residual = -1;
     }
-    std::map<std::tuple<int, int, int, int>, std::vector<Job*>> groups;
-    for (auto& j : jobs) groups[{j.prm.precision, j.prm.solver, j.shape.tx, j.shape.ty}].push_back(&j);
+    // one batched solve per (precision, solver); its block shape is the one chosen for the
+    // largest volume (results do not depend on the tile shape), each volume keeps its own tz
+    std::map<std::pair<int, int>, std::vector<Job*>> groups;
+    for (auto& j : jobs) groups[{j.prm.precision, j.prm.solver}].push_back(&j);
     int worst = FAST_OK;
     for (auto& g : groups) {
-        const int prec = std::get<0>(g.first), solver = std::get<1>(g.first);
-        const Shape sh{std::get<2>(g.first), std::get<3>(g.first), 16};
+        const int prec = g.first.first, solver = g.first.second;
+        const Job* big = g.second[0];
+        for (const Job* j : g.second)
+            if ((size_t)j->vol->nx * j->vol->ny * j->vol->nz > (size_t)big->vol->nx * big->vol->ny * big->vol->nz) big = j;
+        const Shape sh{big->shape.tx, big->shape.ty, 16};
         int rc;
         if (prec == FAST_FP64)
             rc = solve_plain<double, double>(ctx, g.second, sh, solver);
@@ -862,11 +908,11 @@ static int debug_apply(fast_volume* vol, int dir, const T* x, T* y) {
     return hip_ok(ctx, e, "debug apply") ? FAST_OK : FAST_E_HIP;
 }
 
-// Time the three fused passes of BiCGSTAB on one volume: set up the solve, run `iters`
+// Time the fused passes of one solver on one volume: set up the solve, run `iters`
 // iterations with events around every pass (tolerance 0: nothing converges), and return the
-// mean device time per launch of pass 1, 2, 3 and of the whole iteration (ms).
+// mean device time per launch of each pass and of the whole iteration (ms).
 template <class T, class A>
-static int time_passes(fast_volume* vol, int dir, int iters, double* ms) {
+static int time_passes(fast_volume* vol, int dir, int solver, int iters, double* ms) {
     fast_ctx* ctx = vol->ctx;
     std::lock_guard<std::mutex> lk(ctx->mu);
     (void)hipSetDevice(ctx->device);
@@ -874,7 +920,7 @@ static int time_passes(fast_volume* vol, int dir, int iters, double* ms) {
     j.vol = vol;
     fast_tau_params_default(&j.prm);
     j.prm.dir = dir;
-    j.prm.solver = FAST_BICGSTAB;
+    j.prm.solver = solver;
     j.prm.tolerance = 0.0;
     j.prm.max_iter = 1L << 40;
     fast_tau_result res;
@@ -888,22 +934,27 @@ static int time_passes(fast_volume* vol, int dir, int iters, double* ms) {
     int rc = prepare(B);
     if (rc) return rc;
     hipStream_t s = ctx->stream;
-    std::vector<hipEvent_t> ev((size_t)iters * 6);
+    const bool cg = (solver == FAST_CG);
+    const int nev = cg ? 4 : 6;
+    std::vector<hipEvent_t> ev((size_t)iters * nev);
     for (auto& e : ev) (void)hipEventCreate(&e);
-    hipError_t e = Passes<T, A>::init_bicg(B.L, s);
-    for (int i = 0; i < 2 && e == hipSuccess; ++i) e = Passes<T, A>::bicg_iter(B.L, s);   // warm-up
-    for (int i = 0; i < iters && e == hipSuccess; ++i) e = Passes<T, A>::bicg_iter_timed(B.L, s, &ev[(size_t)i * 6]);
+    hipError_t e = cg ? Passes<T, A>::init_cg(B.L, s) : Passes<T, A>::init_bicg(B.L, s);
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) e = cg ? Passes<T, A>::cg_iter(B.L, s) : Passes<T, A>::bicg_iter(B.L, s);
+    for (int i = 0; i < iters && e == hipSuccess; ++i)
+        e = cg ? Passes<T, A>::cg_iter_timed(B.L, s, &ev[(size_t)i * nev])
+               : Passes<T, A>::bicg_iter_timed(B.L, s, &ev[(size_t)i * nev]);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     double acc[4] = {0, 0, 0, 0};
+    const int npass = nev / 2;
     if (e == hipSuccess) {
         for (int i = 0; i < iters; ++i) {
             float t;
-            for (int k = 0; k < 3; ++k) {
-                (void)hipEventElapsedTime(&t, ev[(size_t)i * 6 + 2 * k], ev[(size_t)i * 6 + 2 * k + 1]);
+            for (int k = 0; k < npass; ++k) {
+                (void)hipEventElapsedTime(&t, ev[(size_t)i * nev + 2 * k], ev[(size_t)i * nev + 2 * k + 1]);
                 acc[k] += t;
             }
             if (i + 1 < iters) {
-                (void)hipEventElapsedTime(&t, ev[(size_t)i * 6], ev[(size_t)(i + 1) * 6]);
+                (void)hipEventElapsedTime(&t, ev[(size_t)i * nev], ev[(size_t)(i + 1) * nev]);
                 acc[3] += t;
             }
         }
@@ -917,11 +968,12 @@ static int time_passes(fast_volume* vol, int dir, int iters, double* ms) {
 
 extern "C" {
 
-int fast_debug_time_passes(fast_volume* vol, int dir, int precision, int iters, double* ms) {
-    if (!vol || !ms || iters < 1 || dir < 0 || dir > 5) return FAST_E_BADARG;
-    if (precision == FAST_FP64) return time_passes<double, double>(vol, dir, iters, ms);
-    if (precision == FAST_MIXED) return time_passes<float, double>(vol, dir, iters, ms);
-    if (precision == FAST_FP32) return time_passes<float, float>(vol, dir, iters, ms);
+int fast_debug_time_passes(fast_volume* vol, int dir, int precision, int solver, int iters, double* ms) {
+    if (!vol || !ms || iters < 1 || dir < 0 || dir > 5 || (solver != FAST_BICGSTAB && solver != FAST_CG))
+        return FAST_E_BADARG;
+    if (precision == FAST_FP64) return time_passes<double, double>(vol, dir, solver, iters, ms);
+    if (precision == FAST_MIXED) return time_passes<float, double>(vol, dir, solver, iters, ms);
+    if (precision == FAST_FP32) return time_passes<float, float>(vol, dir, solver, iters, ms);
     return FAST_E_BADARG;
 }
 

@@ -36,6 +36,7 @@ struct fast_ctx {
     hipEvent_t t0 = nullptr, t1 = nullptr;
     unsigned long long* zero_count = nullptr;
     std::map<std::string, fth::Shape> tuned;   // autotuned tile shapes by size class
+    fth::Shape forced{0, 0, 0};                // tile shape forced by fast_ctx_set_tile_shape
     int chunk = 16;                            // Krylov iterations per polling round
     bool use_graphs = true;                    // replay polling rounds from a captured HIP graph
     double mixed_restart_cos = 1e-6;           // mixed form: restart when r0 and r are this close to orthogonal

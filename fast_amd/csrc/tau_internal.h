@@ -114,6 +114,9 @@ struct Passes {
     static hipError_t bicg_iter_timed(const Launch<T, A>&, hipStream_t, hipEvent_t* ev);
     // One PCG iteration (2 fused passes + 2 scalar updates).
     static hipError_t cg_iter(const Launch<T, A>&, hipStream_t);
+    // Same, recording ev[0] / ev[1] around the stencil pass and ev[2] / ev[3] around the
+    // elementwise pass.
+    static hipError_t cg_iter_timed(const Launch<T, A>&, hipStream_t, hipEvent_t* ev);
     // y = A x (diagnostic; same device code as the solver).
     static hipError_t apply(const Launch<T, A>&, const T* x, T* y, hipStream_t);
 };
@@ -129,10 +132,11 @@ struct Passes {
 hipError_t mixed_residual(const Launch<float, double>&, const Coef<double>* coef64, hipStream_t);
 hipError_t mixed_update(const Launch<float, double>&, hipStream_t);
 
-// Flux integral of the concentration `conc` (element offsets VolDesc::off):
-// plane_out[plane_base + k] = (sum over plane k of D (c - c_next) / dx) / n, in T.
+// Flux integral of the concentration `conc` (element offsets VolDesc::off) over a list of
+// planes: entry i is plane plane_k[i] (along the primary axis) of volume plane_vol[i];
+// plane_out[i] = (sum over that plane of D (c - c_next) / dx) / n, in T.
 template <class T>
-hipError_t flux_planes(int total_planes, const VolDesc* vol, const int* plane_vol, const Coef<T>* coef,
+hipError_t flux_planes(int count, const VolDesc* vol, const int* plane_vol, const int* plane_k, const Coef<T>* coef,
                        const T* conc, double* plane_out, hipStream_t);
 
 }  // namespace ft

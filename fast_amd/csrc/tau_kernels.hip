@@ -23,11 +23,11 @@
 // Reductions.  Each tile writes its partial sums (fixed in-block tree); one wave per volume
 // then reduces that volume's tiles in a fixed order and updates the device-resident scalars.
 // Results therefore depend only on the volume and its tile shape, never on what else is in
-// the batch, and are identical run to run.  The FP64 and FP32 solves accumulate every dot
-// product as exact products (fma error term) in double-double and round once, so a dot
-// rounds to the same value whatever the reduction tree: the Krylov trajectory then matches
-// a sequential CPU evaluation of the same recurrences bit for bit.  The mixed form's inner
-// FP32 solve accumulates in plain FP64.
+// the batch, and are identical run to run.  Every dot product is accumulated as exact
+// products (fma error term) in double-double and rounded once, so a dot rounds to the same
+// value whatever the reduction tree: the Krylov trajectory is independent of the tile shape
+// and of the batch, and matches a sequential CPU evaluation of the same recurrences bit for
+// bit.
 #include "tau_internal.h"
 #include <cfloat>
 
@@ -192,22 +192,11 @@ struct AccDD {
     __device__ __forceinline__ void add(float a, float b) { add2((double)a * (double)b, 0.0); }   // exact
 };
 
-// Plain FP64 accumulator (mixed form inner solve); lo stays 0 so partials share the layout.
-struct AccD {
-    double hi, lo;
-    __device__ __forceinline__ void init() { hi = 0.0; lo = 0.0; }
-    __device__ __forceinline__ void add2(double p, double e) { hi += p + e; }
-    __device__ __forceinline__ void add(double a, double b) { hi += a * b; }
-    __device__ __forceinline__ void add(float a, float b) { hi += (double)a * (double)b; }
-};
-
+// Every solve accumulates its dot products in double-double: a dot then rounds to the same
+// value for any reduction tree, so results do not depend on the tile shape or on the batch.
 template <class T, class A>
 struct AccOf {
     using type = AccDD;
-};
-template <>
-struct AccOf<float, double> {
-    using type = AccD;
 };
 
 template <class Acc>
@@ -965,11 +954,15 @@ hipError_t launch_iter_bicg(const Launch<T, A>& L, hipStream_t s, hipEvent_t* ev
 }
 
 template <class T, class A, int TX, int TY>
-hipError_t launch_iter_cg(const Launch<T, A>& L, hipStream_t s) {
+hipError_t launch_iter_cg(const Launch<T, A>& L, hipStream_t s, hipEvent_t* ev = nullptr) {
     const dim3 g(L.ntiles), b(TX * TY);
+    if (ev) (void)hipEventRecord(ev[0], s);
     hipLaunchKernelGGL((k_stencil<T, A, TX, TY, K_CG1>), g, b, 0, s, L, (const T*)nullptr, (T*)nullptr);
+    if (ev) (void)hipEventRecord(ev[1], s);
     hipLaunchKernelGGL((k_fin<T, A, F_CG1>), dim3(L.nvol), dim3(64), 0, s, L);
+    if (ev) (void)hipEventRecord(ev[2], s);
     hipLaunchKernelGGL((k_elem<T, A, TX, TY, E_CG2>), g, b, 0, s, L);
+    if (ev) (void)hipEventRecord(ev[3], s);
     hipLaunchKernelGGL((k_fin<T, A, F_CG2>), dim3(L.nvol), dim3(64), 0, s, L);
     return hipGetLastError();
 }
@@ -1079,6 +1072,14 @@ hipError_t Passes<T, A>::cg_iter(const Launch<T, A>& L, hipStream_t s) {
 }
 
 template <class T, class A>
+hipError_t Passes<T, A>::cg_iter_timed(const Launch<T, A>& L, hipStream_t s, hipEvent_t* ev) {
+#define FT_CALL(TXV, TYV) return launch_iter_cg<T, A, TXV, TYV>(L, s, ev);
+    FT_SHAPES(FT_TRY)
+#undef FT_CALL
+    return hipErrorInvalidValue;
+}
+
+template <class T, class A>
 hipError_t Passes<T, A>::apply(const Launch<T, A>& L, const T* x, T* y, hipStream_t s) {
 #define FT_CALL(TXV, TYV)                                                                                        \
     hipLaunchKernelGGL((k_stencil<T, A, TXV, TYV, K_APPLY>), dim3(L.ntiles), dim3(TXV * TYV), 0, s, L, x, y);    \
@@ -1175,11 +1176,11 @@ __global__ __launch_bounds__(TX* TY) void k_mixres(Launch<float, double> L, cons
     pc.r0 = (float*)L.vs.r0;
     pc.p_cur = (float*)(odd ? L.vs.p[1] : L.vs.p[0]);
     pc.v_cur = (float*)(odd ? L.vs.v[1] : L.vs.v[0]);
-    AccD acc[4];
+    AccDD acc[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) acc[k].init();
-    march<double, MixOps, MixCtx, AccD, 4, TX, TY>(pc, vd, tile - vd.tile_base, lds, acc);
-    block_reduce_store<AccD, 4, TX * TY>(acc, L.part + (long long)tile * kPart);
+    march<double, MixOps, MixCtx, AccDD, 4, TX, TY>(pc, vd, tile - vd.tile_base, lds, acc);
+    block_reduce_store<AccDD, 4, TX * TY>(acc, L.part + (long long)tile * kPart);
 }
 
 // Finalize of a reliable update / fresh start: the FP64 test decides whether the volume is
@@ -1193,7 +1194,7 @@ __global__ __launch_bounds__(64) void k_fin_mixres(Launch<float, double> L) {
     if (status != ST_UPDATE && status != ST_FRESH) return;
     const VolDesc& vd = L.vol[v];
     const int lane = threadIdx.x;
-    AccD sums[4];
+    AccDD sums[4];
     for (int k = 0; k < 4; ++k) sums[k].init();
     for (int t = lane; t < vd.ntiles; t += 64) {
         const double* p = L.part + (long long)(vd.tile_base + t) * kPart;
@@ -1289,13 +1290,13 @@ hipError_t mixed_update(const Launch<float, double>& L, hipStream_t s) {
 // memory order, then the fixed block tree.
 namespace {
 template <class T>
-__global__ __launch_bounds__(256) void k_flux_planes(const VolDesc* vol, const int* plane_vol, const Coef<T>* coef,
-                                                     const T* conc, double* plane_out) {
+__global__ __launch_bounds__(256) void k_flux_planes(const VolDesc* vol, const int* plane_vol, const int* plane_k,
+                                                     const Coef<T>* coef, const T* conc, double* plane_out) {
     const int gp = blockIdx.x;
     const int v = plane_vol[gp];
     const VolDesc& vd = vol[v];
     const Coef<T> cf = coef[v];
-    const int k = gp - vd.plane_base;
+    const int k = plane_k[gp];
     const int nx = vd.nx, ny = vd.ny, nz = vd.nz, p = vd.primary;
     const int dims[3] = {nx, ny, nz};
     const long long n = (long long)dims[(p + 1) % 3] * dims[(p + 2) % 3];
@@ -1333,14 +1334,15 @@ __global__ __launch_bounds__(256) void k_flux_planes(const VolDesc* vol, const i
 }  // namespace
 
 template <class T>
-hipError_t flux_planes(int total_planes, const VolDesc* vol, const int* plane_vol, const Coef<T>* coef, const T* conc,
-                       double* plane_out, hipStream_t s) {
-    hipLaunchKernelGGL((k_flux_planes<T>), dim3(total_planes), dim3(256), 0, s, vol, plane_vol, coef, conc, plane_out);
+hipError_t flux_planes(int count, const VolDesc* vol, const int* plane_vol, const int* plane_k, const Coef<T>* coef,
+                       const T* conc, double* plane_out, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_flux_planes<T>), dim3(count), dim3(256), 0, s, vol, plane_vol, plane_k, coef, conc, plane_out);
     return hipGetLastError();
 }
-template hipError_t flux_planes<double>(int, const VolDesc*, const int*, const Coef<double>*, const double*, double*,
-                                        hipStream_t);
-template hipError_t flux_planes<float>(int, const VolDesc*, const int*, const Coef<float>*, const float*, double*,
-                                       hipStream_t);
+template hipError_t flux_planes<double>(int, const VolDesc*, const int*, const int*, const Coef<double>*, const double*,
+                                        double*, hipStream_t);
+template hipError_t flux_planes<float>(int, const VolDesc*, const int*, const int*, const Coef<float>*, const float*,
+                                       double*, hipStream_t);
 
 }  // namespace ft

@@ -21,7 +21,7 @@ from typing import Optional, Sequence
 
 import numpy as np
 
-from ._lib import (FAST_E_NOCONV, FAST_OK, FastError, ServerConfig, TauParams, TauResult, check, lib)
+from ._lib import (FAST_E_BUSY, FAST_E_NOCONV, FAST_OK, FastError, ServerConfig, TauParams, TauResult, check, lib)
 
 
 class Dir(enum.IntEnum):
@@ -315,6 +315,57 @@ def solve_batch(ctx: Context, vols: Sequence[Volume], params: Sequence[TauParams
     if rc not in (FAST_OK, FAST_E_NOCONV):
         check(rc, "fast_tau_solve_batch", ctx.handle)
     return [arr_r[i] for i in range(n)]
+
+
+class Server:
+    """Continuous-batching tau solves: requests of any size join the running batch at round
+    boundaries and leave when their FP64 true residual meets the tolerance (mixed form).
+    The server owns `ctx` while it is open."""
+
+    def __init__(self, ctx: Context, hbm_budget: int = 0, max_active: int = 0, iters_per_round: int = 0,
+                 use_bicgstab: bool = False):
+        cfg = ServerConfig(hbm_budget=hbm_budget, max_active=max_active, iters_per_round=iters_per_round,
+                           use_bicgstab=int(use_bicgstab))
+        self.ctx = ctx
+        self._h = C.c_void_p()
+        check(lib().fast_server_create(ctx.handle, C.byref(cfg), C.byref(self._h)), "fast_server_create", ctx.handle)
+
+    def submit(self, mask: np.ndarray, params: TauParams) -> int:
+        m, nx, ny, nz = _mask_xyz(mask)
+        t = C.c_uint64()
+        check(lib().fast_server_submit(self._h, _u8(m), nx, ny, nz, C.byref(params), C.byref(t)),
+              "fast_server_submit", self.ctx.handle)
+        return int(t.value)
+
+    def wait(self, timeout_ms: int = -1):
+        """(ticket, TauResult, latency_ms) of the next finished request, or None on timeout."""
+        t, res, lat = C.c_uint64(), TauResult(), C.c_double()
+        rc = lib().fast_server_wait(self._h, timeout_ms, C.byref(t), C.byref(res), C.byref(lat))
+        if rc == FAST_E_BUSY:
+            return None
+        if rc not in (FAST_OK, FAST_E_NOCONV):
+            check(rc, "fast_server_wait", self.ctx.handle)
+        return int(t.value), res, float(lat.value)
+
+    def pending(self) -> int:
+        return int(lib().fast_server_pending(self._h))
+
+    def close(self):
+        if self._h:
+            lib().fast_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---- volumes: .bin I/O and generators (host) -----------------------------------------------

@@ -98,6 +98,13 @@ void fast_ctx_destroy(fast_ctx* ctx);
 const char* fast_ctx_last_error(const fast_ctx* ctx);
 /* Bytes of device memory currently held by the context. */
 size_t fast_ctx_device_bytes(const fast_ctx* ctx);
+/* Force the tile shape of subsequent solves: a block of tx*ty threads owns a tx*ty column
+ * brick and marches tz planes (tx,ty one of 64x8, 64x4, 32x8, 32x16, 16x16; tz 1..256).
+ * tx = 0 restores the automatic choice (the autotuned table, else a size rule).  Used by
+ * the autotuner (fast_amd/autotune.py).  No reference counterpart. */
+int fast_ctx_set_tile_shape(fast_ctx* ctx, int tx, int ty, int tz);
+/* Polling-round length in Krylov iterations (captured as one HIP graph); 0 = default 16. */
+int fast_ctx_set_round(fast_ctx* ctx, int iterations);
 
 /* ---- device-resident volumes ---------------------------------------------- */
 /* A mask uploaded once and kept in HBM with its per-voxel stencil code; several solves
@@ -153,13 +160,20 @@ int fast_ccl_boundary_table(fast_volume* vol, uint8_t background, uint8_t* table
  * and leave on convergence.  No reference counterpart. */
 typedef struct fast_server fast_server;
 typedef struct fast_server_config {
-    size_t hbm_budget;      /* bytes for resident masks + Krylov state (0 = context budget) */
+    size_t hbm_budget;      /* bytes of the Krylov-state arena, 40 B per voxel
+                             (0 = half the context budget, at most 16 GiB) */
     int max_active;         /* max concurrently solving requests (0 = 64) */
     int iters_per_round;    /* Krylov iterations between admission points (0 = 8) */
-    int reserved[4];
+    int use_bicgstab;       /* 0: CG (default), 1: BiCGSTAB */
+    int reserved[3];
 } fast_server_config;
 
 int fast_server_create(fast_ctx* ctx, const fast_server_config* cfg, fast_server** out);
+/* Requests are solved in the mixed form (FP32 vectors, FP64 arithmetic, reliable updates) to
+ * params->tolerance on the FP64 true residual; params->solver and ->precision are ignored.
+ * A request's result is bit-identical to fast_tau_solve_volume on the same volume with
+ * precision FAST_MIXED and the server's solver, whatever else is in flight.  The server owns
+ * its context while it runs: do not use the context from other threads meanwhile. */
 /* Copy the mask and queue a request; returns a ticket.  Never blocks on the GPU. */
 int fast_server_submit(fast_server* srv, const uint8_t* mask, int nx, int ny, int nz,
                        const fast_tau_params* params, uint64_t* ticket);
@@ -194,11 +208,11 @@ int fast_debug_apply_f64(fast_volume* vol, int dir, const double* x, double* y);
 int fast_debug_apply_f32(fast_volume* vol, int dir, const float* x, float* y);
 /* Right-hand side b, initial guess x0 and Jacobi inverse 1/diag(A) (FP64). */
 int fast_debug_system_f64(fast_volume* vol, int dir, double* b, double* x0, double* invdiag);
-/* Mean device time (ms, HIP events on the solve stream) per launch of the three fused
- * BiCGSTAB passes and per iteration, over `iters` iterations of a solve set up on `vol`
- * (precision FAST_FP64, FAST_FP32, or FAST_MIXED for the FP32-vector / FP64-reduction
- * inner passes).  ms: 4 doubles {pass1, pass2, pass3, iteration}. */
-int fast_debug_time_passes(fast_volume* vol, int dir, int precision, int iters, double* ms);
+/* Mean device time (ms, HIP events on the solve stream) per launch of each fused pass and
+ * per iteration, over `iters` iterations of a solve set up on `vol` (precision FAST_FP64,
+ * FAST_FP32, or FAST_MIXED for the FP32-storage / FP64-arithmetic passes).  ms: 4 doubles;
+ * BiCGSTAB {pass1, pass2, pass3, iteration}, CG {stencil pass, update pass, 0, iteration}. */
+int fast_debug_time_passes(fast_volume* vol, int dir, int precision, int solver, int iters, double* ms);
 
 #ifdef __cplusplus
 }
