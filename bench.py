@@ -77,6 +77,55 @@ def cpu_baseline(mask: np.ndarray, gpu_iters: float, threads: int, sample_iters:
     }
 
 
+def serving(ctx, T, W, n_req: int, loads=(0.5, 0.8, 1.1)) -> dict:
+    """Continuous-batching server under Poisson arrivals.  Requests: cubic edge from
+    {64,128,192,256,384} (weights .3/.3/.2/.15/.05), direction uniform in x/y/z, ellipsoid
+    volumes (r log-uniform on [0.1, 10], solid 0.5..0.75) from a pool of 4 per edge; mixed CG
+    to tol 1e-6.  The saturated rate is measured with a burst; then `n_req` requests arrive at
+    each offered load (fraction of that rate).  Latency = completion - submission."""
+    import threading
+    pool = W.request_pool(4)
+    mk = lambda d: T.make_params(d, CG, "mixed", TOL, 20000)
+    out = {"requests_per_load": n_req, "volume_pool": "4 volumes per edge", "tolerance": TOL, "solver": "mixed CG"}
+    with T.Server(ctx, max_active=64, iters_per_round=16) as srv:
+        burst = W.request_stream(max(n_req // 2, 24), 1e9, seed=11)
+        t0 = time.perf_counter()
+        for q in burst:
+            srv.submit(pool[(q.edge, q.volume)], mk(q.dir))
+        for _ in burst:
+            srv.wait(-1)
+        sat = len(burst) / (time.perf_counter() - t0)
+        out["saturated_solves_per_s"] = sat
+        log(f"[bench] serving: saturated {sat:.1f} solves/s over a burst of {len(burst)}")
+        runs = []
+        for li, f in enumerate(loads):
+            reqs = W.request_stream(n_req, f * sat, seed=100 + li)
+            lat = []
+            start = time.perf_counter()
+
+            def feed():
+                for q in reqs:
+                    dt = start + q.arrival - time.perf_counter()
+                    if dt > 0:
+                        time.sleep(dt)
+                    srv.submit(pool[(q.edge, q.volume)], mk(q.dir))
+
+            th = threading.Thread(target=feed)
+            th.start()
+            for _ in reqs:
+                _, res, ms = srv.wait(-1)
+                lat.append(ms)
+            th.join()
+            wall = time.perf_counter() - start
+            runs.append({"offered_fraction": f, "offered_solves_per_s": f * sat, "achieved_solves_per_s": n_req / wall,
+                         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                         "mean_ms": float(np.mean(lat))})
+            log(f"[bench] serving load {f:.1f}: p50 {runs[-1]['p50_ms']:.0f} ms p99 {runs[-1]['p99_ms']:.0f} ms "
+                f"{runs[-1]['achieved_solves_per_s']:.1f} solves/s")
+        out["loads"] = runs
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +134,7 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-iters", type=int, default=25)
+    ap.add_argument("--serve", type=int, default=120, help="requests per offered load (0: skip serving)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -93,6 +143,7 @@ def main() -> int:
 
     import fast_amd as F
     from fast_amd import tau as T
+    from fast_amd import workloads as W
     from fast_amd._lib import lib
 
     n = args.size
@@ -140,13 +191,13 @@ def main() -> int:
     fp64_cg_pass = list(ms)
     lib().fast_debug_time_passes(vol.handle, 0, 0, BICG, 20, ms)
     fp64_bicg_pass = list(ms)
-    # mixed CG: stencil pass reads r, p_old and the code byte, writes p, q (4 FP32 vectors);
-    # update pass reads x, p, r, q and the code byte, writes x, r (6 FP32 vectors)
+    # mixed CG: pass 1 reads r, p_old and the code byte and writes p (3 FP32 vectors);
+    # pass 2 reads p (stencil), x, r and the code byte and writes x, r (5 FP32 vectors)
     kernels = {
-        "k_stencil<float,double,64,8,CG1> (PCG stencil pass: p = M^-1 r + beta p, q = A p, p.q)":
-            (mixed_pass[0], (4 * 4 + 1) * nvox),
-        "k_elem<float,double,64,8,CG2> (PCG update pass: x, r, r.r, r.M^-1 r)":
-            (mixed_pass[1], (6 * 4 + 1) * nvox),
+        "k_stencil<float,double,64,8,K_CG1> (PCG pass 1: p = M^-1 r + beta p, p.Ap)":
+            (mixed_pass[0], (3 * 4 + 1) * nvox),
+        "k_stencil<float,double,64,8,K_CG2> (PCG pass 2: q = Ap, x += alpha p, r -= alpha q, r.r, r.M^-1 r)":
+            (mixed_pass[1], (5 * 4 + 1) * nvox),
     }
     name, (kms, kbytes) = max(kernels.items(), key=lambda kv: kv[1][0])
     ach = kbytes / (kms * 1e-3) / 1e9
@@ -158,6 +209,11 @@ def main() -> int:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         log(f"[bench] CPU oracle sample on {threads} threads")
         cpu = cpu_baseline(mask, float(fp64[(CG, 0)][0].iterations), threads, args.cpu_iters)
+
+    serve = None
+    if args.serve > 0:
+        vol.close()
+        serve = serving(ctx, T, W, args.serve)
 
     best_fp64 = max(fp64_rate.values())
     line = {
@@ -202,6 +258,7 @@ def main() -> int:
         "kernels_mixed_cg": others,
         "pass_ms": {"mixed_cg": mixed_pass, "fp64_cg": fp64_cg_pass, "fp64_bicgstab": fp64_bicg_pass},
         "cpu_baseline": cpu,
+        "serving": serve,
     }
     print(json.dumps(line), flush=True)
     vol.close()

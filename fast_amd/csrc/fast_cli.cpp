@@ -199,7 +199,6 @@ int main(int argc, char** argv) {
     const bool single = (prec == "float" || prec == "single" || prec == "f" || prec == "s");
     const bool mixed = (prec == "mixed" || prec == "m");
     p.precision = mixed ? FAST_MIXED : (single ? FAST_FP32 : FAST_FP64);
-    if (mixed) p.solver = FAST_BICGSTAB;
     p.tolerance = std::pow(10.0, -tol);
     p.max_iter = iters;
     p.porosity = porosity;

@@ -37,6 +37,7 @@ double now_s() {
 
 constexpr int kTX = 64, kTY = 8;      // one launch shape for the whole live set
 constexpr size_t kAlign = 64;         // arena granularity (elements)
+constexpr size_t kPlanes = 8;         // FP32 vectors per request: x (the correction), r, r0, p[2], v[2], s
 
 struct Req {
     uint64_t ticket = 0;
@@ -245,7 +246,7 @@ int rebuild(fast_server* s, const std::vector<Req*>& adm) {
     L.vs.p[1] = a + 4 * S;
     L.vs.v[0] = a + 5 * S;
     L.vs.v[1] = a + 6 * S;
-    L.vs.t = a + 7 * S;
+    L.vs.s = a + 7 * S;
     L.vs.x64 = (double*)(a + 8 * S);
     return FAST_OK;
 }
@@ -393,8 +394,9 @@ void worker_main(fast_server* s) {
         for (int i = 0; i < m && e == hipSuccess; ++i) {
             e = (s->solver == FAST_BICGSTAB) ? Passes<float, double>::bicg_iter(s->L, st)
                                              : Passes<float, double>::cg_iter(s->L, st);
-            if (e == hipSuccess) e = ft::mixed_update(s->L, st);
-            if (e == hipSuccess) e = ft::mixed_residual(s->L, s->d_coef, st);
+            const bool upd = (i + 1) % kMixedUpdateEvery == 0 || i + 1 == m;
+            if (e == hipSuccess && upd) e = ft::mixed_update(s->L, st);
+            if (e == hipSuccess && upd) e = ft::mixed_residual(s->L, s->d_coef, st);
         }
         if (e == hipSuccess)
             e = hipMemcpyAsync(s->h_st, s->d_st, sizeof(VolState<double>) * s->max_active, hipMemcpyDeviceToHost, st);
@@ -434,8 +436,8 @@ int fast_server_create(fast_ctx* ctx, const fast_server_config* cfg, fast_server
     s->max_active = s->cfg.max_active;
     const size_t budget = s->cfg.hbm_budget ? s->cfg.hbm_budget : std::min(ctx->budget / 2, (size_t)16 << 30);
     // per element: 8 FP32 planes + the FP64 solution
-    s->S = align_up(budget / (8 * sizeof(float) + sizeof(double)), kAlign);
-    s->arena_mem = dalloc(ctx, s->S * (8 * sizeof(float) + sizeof(double)));
+    s->S = align_up(budget / (kPlanes * sizeof(float) + sizeof(double)), kAlign);
+    s->arena_mem = dalloc(ctx, s->S * (kPlanes * sizeof(float) + sizeof(double)));
     s->d_vol = (VolDesc*)dalloc(ctx, sizeof(VolDesc) * s->max_active);
     s->d_coef = (Coef<double>*)dalloc(ctx, sizeof(Coef<double>) * s->max_active);
     s->d_st = (VolState<double>*)dalloc(ctx, sizeof(VolState<double>) * s->max_active);

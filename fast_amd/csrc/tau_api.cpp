@@ -266,6 +266,7 @@ int fast_ctx_create(int device, size_t hbm_budget, fast_ctx** out) {
     }
     load_tuning(ctx);
     if (const char* e = std::getenv("FAST_TAU_MIXED_RESTART")) ctx->mixed_restart_cos = std::atof(e);
+    if (const char* e = std::getenv("FAST_TAU_GRAPHS")) ctx->use_graphs = std::atoi(e) != 0;
     *out = ctx;
     return FAST_OK;
 }
@@ -536,7 +537,7 @@ static int prepare(Batch<T, A>& B) {
     o = align_up(o + sizeof(double) * planes, 256);
     const size_t meta_bytes = o;
     B.meta = dalloc(ctx, meta_bytes);
-    const int nvec = 8;
+    const int nvec = 8;   // x, r, r0, p[2], v[2], s
     const size_t vec_bytes = nvec * total * sizeof(T) + (B.mixed ? total * sizeof(double) : 0);
     B.vecs = dalloc(ctx, vec_bytes);
     if (!B.meta || !B.vecs) return FAST_E_OOM;
@@ -574,7 +575,7 @@ static int prepare(Batch<T, A>& B) {
     vs.p[1] = vb + 4 * total;
     vs.v[0] = vb + 5 * total;
     vs.v[1] = vb + 6 * total;
-    vs.t = vb + 7 * total;
+    vs.s = vb + 7 * total;
     vs.x64 = B.mixed ? (double*)(vb + 8 * total) : nullptr;
     return FAST_OK;
 }
@@ -754,8 +755,11 @@ static int solve_mixed(fast_ctx* ctx, std::vector<Job*>& jobs, Shape shape, int 
         hipError_t e2 = hipSuccess;
         for (int i = 0; i < chunk && e2 == hipSuccess; ++i) {
             e2 = (solver == FAST_BICGSTAB) ? Passes<float, double>::bicg_iter(B.L, s) : Passes<float, double>::cg_iter(B.L, s);
-            if (e2 == hipSuccess) e2 = ft::mixed_update(B.L, s);
-            if (e2 == hipSuccess) e2 = ft::mixed_residual(B.L, B.coef64, s);
+            // A volume that asked for an update skips the iterations until the update step
+            // runs, so running it every few iterations changes nothing but the idle launches.
+            const bool upd = (i + 1) % kMixedUpdateEvery == 0 || i + 1 == chunk;
+            if (e2 == hipSuccess && upd) e2 = ft::mixed_update(B.L, s);
+            if (e2 == hipSuccess && upd) e2 = ft::mixed_residual(B.L, B.coef64, s);
         }
         return e2;
     };
@@ -900,7 +904,7 @@ static int debug_apply(fast_volume* vol, int dir, const T* x, T* y) {
     if (rc) return rc;
     const size_t n = (size_t)vol->nx * vol->ny * vol->nz;
     T* dx = (T*)B.L.vs.x;
-    T* dy = (T*)B.L.vs.t;
+    T* dy = (T*)B.L.vs.s;
     hipError_t e = hipMemcpyAsync(dx, x, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = Passes<T, T>::apply(B.L, dx, dy, ctx->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(y, dy, n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream);

@@ -11,6 +11,11 @@
 
 namespace fth {
 
+// Mixed form: the reliable-update step is launched after every kMixedUpdateEvery iterations
+// and at the end of every polling chunk.  A volume waiting for its update skips the iterations in
+// between, so its trajectory does not depend on this interval.
+constexpr int kMixedUpdateEvery = 4;
+
 struct Shape {
     int tx, ty, tz;
 };

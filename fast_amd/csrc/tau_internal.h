@@ -64,7 +64,7 @@ struct VecSet {
     void* r0;
     void* p[2];
     void* v[2];
-    void* t;
+    void* s;       // BiCGSTAB's s = r - alpha v; output of the apply diagnostic
     double* x64;   // mixed refinement: FP64 solution
 };
 

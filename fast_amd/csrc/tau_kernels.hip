@@ -43,11 +43,12 @@ enum PassKind : int {
     K_RESTART,
     K_BICG1,
     K_BICG2,
+    K_BICG3,
     K_CG1,
+    K_CG2,
     K_APPLY,
     K_MIXRES,
 };
-enum ElemKind : int { E_BICG3 = 0, E_CG2, E_MIXUPD };
 enum FinKind : int {
     F_INIT_BICG = 0,
     F_RESTART,
@@ -245,13 +246,13 @@ __device__ __forceinline__ void block_reduce_store(Acc (&acc)[NP], double* out) 
 // ---------------------------------------------------------------------------------------
 template <class T>
 struct Raw {
-    T a, b, c, e;
+    T a, b, c, e, f, g;
     int code;
 };
 
 template <class T>
 struct Pt {
-    T u, aux, e, dg;
+    T u, aux, e, f, g, dg;
     int code;
 };
 
@@ -273,7 +274,7 @@ struct PassCtx {
     T* p_new;   // p of this iteration (written)
     T* v_old;
     T* v_new;
-    T* t;
+    T* s;       // BiCGSTAB's s = r - alpha v (pass 2 -> pass 3); the apply diagnostic's output
 };
 
 template <class T, class A, int KIND>
@@ -396,7 +397,7 @@ struct PassOps<T, A, K_APPLY> {
     }
     template <class X, class Acc>
     __device__ static void epi(const X& pc, long long idx, int k, C Au, const Pt<C>& pt, Acc (&)[1]) {
-        pc.t[idx] = T(Au);
+        pc.s[idx] = T(Au);
     }
 };
 
@@ -435,7 +436,9 @@ struct PassOps<T, A, K_BICG1> {
     }
 };
 
-// BICG2: s = r - alpha v; u = M^-1 s; t = A u; partials t.s, t.t
+// BICG2: s = r - alpha v (stored); u = M^-1 s; t = A u; partials t.s, t.t.  t is not stored:
+// BICG3 recomputes it from s.  s is rounded to the storage type first, so that both passes
+// see the same s.
 template <class T, class A>
 struct PassOps<T, A, K_BICG2> {
     using C = typename CmpOf<T, A>::type;
@@ -452,7 +455,7 @@ struct PassOps<T, A, K_BICG2> {
     __device__ static Pt<C> compute(const X& pc, const Raw<C>& w, int bx, int k) {
         Pt<C> q;
         q.code = w.code;
-        const C s = w.a - pc.alpha * w.b;
+        const C s = C(T(w.a - pc.alpha * w.b));
         q.dg = diag_of<C>(w.code, bx, pc.primary, pc.c);
         q.u = (C(1) / q.dg) * s;
         q.aux = s;
@@ -460,13 +463,58 @@ struct PassOps<T, A, K_BICG2> {
     }
     template <class X, class Acc>
     __device__ static void epi(const X& pc, long long idx, int k, C Au, const Pt<C>& pt, Acc (&acc)[NP]) {
-        pc.t[idx] = T(Au);
+        pc.s[idx] = T(pt.aux);
         acc[0].add(Au, pt.aux);
         acc[1].add(Au, Au);
     }
 };
 
-// CG1: z = M^-1 r; p = z (first iteration) or z + beta p_old; q = A p; partial p.q
+// BICG3: t = A M^-1 s (recomputed from the stored s); x += alpha M^-1 p + omega M^-1 s;
+// r = s - omega t; partials r.r, r0.r of the stored residual.  The stencil input is s, never
+// r, so writing r in place cannot race with a neighbouring tile's halo reads.
+template <class T, class A>
+struct PassOps<T, A, K_BICG3> {
+    using C = typename CmpOf<T, A>::type;
+    static constexpr int NP = 2;
+    template <class X>
+    __device__ static Raw<C> fetch(const X& pc, long long idx, long long lin, bool centre) {
+        Raw<C> w;
+        w.a = C(pc.s[idx]);
+        if (centre) {
+            w.c = C(pc.p_new[idx]);
+            w.e = C(pc.r0[idx]);
+            w.f = C(pc.x[idx]);
+        }
+        w.code = pc.code[lin];
+        return w;
+    }
+    template <class X>
+    __device__ static Pt<C> compute(const X& pc, const Raw<C>& w, int bx, int k) {
+        Pt<C> q;
+        q.code = w.code;
+        q.dg = diag_of<C>(w.code, bx, pc.primary, pc.c);
+        q.u = (C(1) / q.dg) * w.a;
+        q.aux = w.a;
+        q.e = w.e;
+        q.f = w.f;
+        q.g = w.c;
+        return q;
+    }
+    template <class X, class Acc>
+    __device__ static void epi(const X& pc, long long idx, int k, C Au, const Pt<C>& pt, Acc (&acc)[NP]) {
+        const C inv = C(1) / pt.dg;
+        const C yv = inv * pt.g;
+        pc.x[idx] = T(pt.f + (pc.alpha * yv + pc.omega * pt.u));
+        const T rs = T(pt.aux - pc.omega * Au);   // the stored residual is what the next iteration sees
+        pc.r[idx] = rs;
+        acc[0].add(C(rs), C(rs));
+        acc[1].add(pt.e, C(rs));
+    }
+};
+
+// CG1: z = M^-1 r; p = z (first iteration) or z + beta p_old; partial p.(A p).  A p is not
+// stored: CG2 recomputes it from the stored p, which costs no extra HBM traffic (CG2 reads p
+// anyway) and saves writing and re-reading q.
 template <class T, class A>
 struct PassOps<T, A, K_CG1> {
     using C = typename CmpOf<T, A>::type;
@@ -492,8 +540,44 @@ struct PassOps<T, A, K_CG1> {
     template <class X, class Acc>
     __device__ static void epi(const X& pc, long long idx, int k, C Au, const Pt<C>& pt, Acc (&acc)[NP]) {
         pc.p_new[idx] = T(pt.u);
-        pc.t[idx] = T(Au);
         acc[0].add(pt.u, Au);
+    }
+};
+
+// CG2: q = A p (recomputed from the stored p); x += alpha p; r -= alpha q; partials r.r and
+// r.(M^-1 r) of the stored residual
+template <class T, class A>
+struct PassOps<T, A, K_CG2> {
+    using C = typename CmpOf<T, A>::type;
+    static constexpr int NP = 2;
+    template <class X>
+    __device__ static Raw<C> fetch(const X& pc, long long idx, long long lin, bool centre) {
+        Raw<C> w;
+        w.a = C(pc.p_new[idx]);
+        if (centre) {
+            w.b = C(pc.x[idx]);
+            w.c = C(pc.r[idx]);
+        }
+        w.code = pc.code[lin];
+        return w;
+    }
+    template <class X>
+    __device__ static Pt<C> compute(const X& pc, const Raw<C>& w, int bx, int k) {
+        Pt<C> q;
+        q.code = w.code;
+        q.dg = diag_of<C>(w.code, bx, pc.primary, pc.c);
+        q.u = w.a;
+        q.aux = w.b;
+        q.e = w.c;
+        return q;
+    }
+    template <class X, class Acc>
+    __device__ static void epi(const X& pc, long long idx, int k, C Au, const Pt<C>& pt, Acc (&acc)[NP]) {
+        pc.x[idx] = T(pt.aux + pc.alpha * pt.u);
+        const T rs = T(pt.e - pc.alpha * Au);   // the stored residual is what the next iteration sees
+        pc.r[idx] = rs;
+        acc[0].add(C(rs), C(rs));
+        acc[1].add(C(rs), (C(1) / pt.dg) * C(rs));
     }
 };
 
@@ -636,10 +720,10 @@ __global__ __launch_bounds__(TX* TY) void k_stencil(Launch<T, A> L, const T* xin
         pc.p_new = (T*)(odd ? L.vs.p[0] : L.vs.p[1]);
         pc.v_old = (T*)(odd ? L.vs.v[1] : L.vs.v[0]);
         pc.v_new = (T*)(odd ? L.vs.v[0] : L.vs.v[1]);
-        pc.t = (T*)L.vs.t;
+        pc.s = (T*)L.vs.s;
         if (KIND == K_APPLY) {
             pc.x = (T*)xin;
-            pc.t = yout;
+            pc.s = yout;
         }
     }
     using Acc = typename AccOf<T, A>::type;
@@ -653,81 +737,32 @@ __global__ __launch_bounds__(TX* TY) void k_stencil(Launch<T, A> L, const T* xin
 // ---------------------------------------------------------------------------------------
 // Elementwise passes over the same tiles (no halo).
 // ---------------------------------------------------------------------------------------
-template <class T, class A, int TX, int TY, int KIND>
-__global__ __launch_bounds__(TX* TY) void k_elem(Launch<T, A> L) {
+// Mixed form, reliable update of the volumes in ST_UPDATE / ST_FRESH: x64 += d (unless the
+// correction is discarded), d = 0.  Elementwise over the same tiles (no halo).
+template <int TX, int TY>
+__global__ __launch_bounds__(TX* TY) void k_mixupd(Launch<float, double> L) {
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
     if (tile >= L.ntiles) return;
     const int v = L.tile_vol[tile];
     const VolDesc& vd = L.vol[v];
-    const VolState<A>* st = L.st + v;
-    if (KIND == E_MIXUPD) {
-        if (st->status != ST_UPDATE && st->status != ST_FRESH) return;
-    } else if (st->status != ST_RUNNING) {
-        return;
-    }
-    using C = typename CmpOf<T, A>::type;
-    const Coef<C> cf = L.coef[v];
-    const bool odd = (st->phase & 1) != 0;
-    const T* Pn = (const T*)(odd ? L.vs.p[0] : L.vs.p[1]);   // p written by this iteration
-    const T* Vn = (const T*)(odd ? L.vs.v[0] : L.vs.v[1]);
-    const int nx = vd.nx, ny = vd.ny, nz = vd.nz, primary = vd.primary;
+    const VolState<double>* st = L.st + v;
+    if (st->status != ST_UPDATE && st->status != ST_FRESH) return;
+    const bool keep = st->refinements > 0 && !st->discard;
+    const int nx = vd.nx, ny = vd.ny, nz = vd.nz;
     const int lt = tile - vd.tile_base;
     const int tix = lt % vd.tiles_x;
     const int tiy = (lt / vd.tiles_x) % vd.tiles_y;
     const int tiz = lt / (vd.tiles_x * vd.tiles_y);
     const int x = tix * TX + (int)(threadIdx.x % TX), y = tiy * TY + (int)(threadIdx.x / TX);
     const int z0 = tiz * vd.tz, z1 = min(z0 + vd.tz, nz);
-    const bool inside = (x < nx) && (y < ny);
+    if (x >= nx || y >= ny) return;
     const long long plane = (long long)nx * ny;
-
-    using Acc = typename AccOf<T, A>::type;
-    Acc acc[2];
-    acc[0].init();
-    acc[1].init();
-    if (inside) {
-        const A alpha = st->alpha, omega = st->omega;
-        for (int z = z0; z < z1; ++z) {
-            const long long lin = (long long)x + (long long)nx * y + plane * z;
-            const long long i = vd.off + lin;
-            if (KIND == E_MIXUPD) {
-                float* D = (float*)L.vs.x;
-                if (st->refinements > 0 && !st->discard) L.vs.x64[i] = L.vs.x64[i] + (double)D[i];
-                D[i] = 0.0f;
-                continue;
-            }
-            const int code = vd.code[lin];
-            const int bx = box_flags(x, y, z, nx, ny, nz);
-            const C inv = C(1) / diag_of<C>(code, bx, primary, cf);
-            T* X = (T*)L.vs.x;
-            T* R = (T*)L.vs.r;
-            if (KIND == E_BICG3) {
-                const C p = C(Pn[i]);
-                const C vv = C(Vn[i]);
-                const C t = C(((const T*)L.vs.t)[i]);
-                const C r0 = C(((const T*)L.vs.r0)[i]);
-                const C r = C(R[i]);
-                const C yv = inv * p;
-                const C s = r - C(alpha) * vv;
-                const C zv = inv * s;
-                X[i] = T(C(X[i]) + (C(alpha) * yv + C(omega) * zv));
-                const C rn = s - C(omega) * t;
-                const T rs = T(rn);   // the stored residual is what the next iteration sees
-                R[i] = rs;
-                acc[0].add(C(rs), C(rs));
-                acc[1].add(r0, C(rs));
-            } else {   // E_CG2
-                const C p = C(Pn[i]);
-                const C q = C(((const T*)L.vs.t)[i]);
-                X[i] = T(C(X[i]) + C(alpha) * p);
-                const C rn = C(R[i]) - C(alpha) * q;
-                const T rs = T(rn);
-                R[i] = rs;
-                acc[0].add(C(rs), C(rs));
-                acc[1].add(C(rs), inv * C(rs));
-            }
-        }
+    float* D = (float*)L.vs.x;
+    for (int z = z0; z < z1; ++z) {
+        const long long i = vd.off + (long long)x + (long long)nx * y + plane * z;
+        if (keep) L.vs.x64[i] = L.vs.x64[i] + (double)D[i];
+        D[i] = 0.0f;
     }
-    if (KIND != E_MIXUPD) block_reduce_store<Acc, 2, TX * TY>(acc, L.part + (long long)tile * kPart);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -947,7 +982,7 @@ hipError_t launch_iter_bicg(const Launch<T, A>& L, hipStream_t s, hipEvent_t* ev
     if (ev) (void)hipEventRecord(ev[3], s);
     hipLaunchKernelGGL((k_fin<T, A, F_BICG2>), dim3(L.nvol), dim3(64), 0, s, L);
     if (ev) (void)hipEventRecord(ev[4], s);
-    hipLaunchKernelGGL((k_elem<T, A, TX, TY, E_BICG3>), g, b, 0, s, L);
+    hipLaunchKernelGGL((k_stencil<T, A, TX, TY, K_BICG3>), g, b, 0, s, L, (const T*)nullptr, (T*)nullptr);
     if (ev) (void)hipEventRecord(ev[5], s);
     hipLaunchKernelGGL((k_fin<T, A, F_BICG3>), dim3(L.nvol), dim3(64), 0, s, L);
     return hipGetLastError();
@@ -961,7 +996,7 @@ hipError_t launch_iter_cg(const Launch<T, A>& L, hipStream_t s, hipEvent_t* ev =
     if (ev) (void)hipEventRecord(ev[1], s);
     hipLaunchKernelGGL((k_fin<T, A, F_CG1>), dim3(L.nvol), dim3(64), 0, s, L);
     if (ev) (void)hipEventRecord(ev[2], s);
-    hipLaunchKernelGGL((k_elem<T, A, TX, TY, E_CG2>), g, b, 0, s, L);
+    hipLaunchKernelGGL((k_stencil<T, A, TX, TY, K_CG2>), g, b, 0, s, L, (const T*)nullptr, (T*)nullptr);
     if (ev) (void)hipEventRecord(ev[3], s);
     hipLaunchKernelGGL((k_fin<T, A, F_CG2>), dim3(L.nvol), dim3(64), 0, s, L);
     return hipGetLastError();
@@ -1275,7 +1310,7 @@ hipError_t mixed_residual(const Launch<float, double>& L, const Coef<double>* co
 
 hipError_t mixed_update(const Launch<float, double>& L, hipStream_t s) {
 #define FT_CALL(TXV, TYV)                                                                                    \
-    hipLaunchKernelGGL((k_elem<float, double, TXV, TYV, E_MIXUPD>), dim3(L.ntiles), dim3(TXV * TYV), 0, s, L); \
+    hipLaunchKernelGGL((k_mixupd<TXV, TYV>), dim3(L.ntiles), dim3(TXV * TYV), 0, s, L); \
     return hipGetLastError();
     FT_SHAPES(FT_TRY)
 #undef FT_CALL

@@ -91,7 +91,10 @@ typedef struct fast_ctx fast_ctx;
 
 /* Create a context on HIP device `device`.  `hbm_budget` (bytes) bounds the device memory
  * the context may hold (0 = 90% of free memory).  Replaces the reference's implicit global
- * CUDA state (Diffusion.cu:8 __constant__ params, default stream). */
+ * CUDA state (Diffusion.cu:8 __constant__ params, default stream).
+ * Environment: FAST_TAU_GRAPHS=0 launches every polling round directly instead of replaying
+ * a captured HIP graph (same kernels, same results); FAST_TAU_TUNING names a tile-shape table;
+ * FAST_TAU_MIXED_RESTART sets the mixed form's shadow-residual reset threshold. */
 int fast_ctx_create(int device, size_t hbm_budget, fast_ctx** out);
 void fast_ctx_destroy(fast_ctx* ctx);
 /* Last error message of this context (never NULL). */

@@ -1,0 +1,72 @@
+"""96 uneven volumes (edges 32..192 per axis, r log-uniform on [0.1, 10], solid 0.5..0.75)
+solved as one batch.  Per-volume tau of the mixed form against the converged FP64 tau (CG to
+1e-11; the FP64 path is bit-identical to the oracle's recurrences), and the FP64 batch
+against solo solves."""
+import numpy as np
+import pytest
+
+from fast_amd import tau as T
+from fast_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+BICG, CG = 0, 1
+
+
+@pytest.fixture(scope="module")
+def batch96(ctx):
+    items = W.uneven_batch(96, 1000)
+    vols = [ctx.volume(b.mask) for b in items]
+    dirs = [(0, 2, 4)[i % 3] for i in range(len(vols))]
+    print("config4: converged FP64 batch", flush=True)
+    conv = T.solve_batch(ctx, vols, [T.make_params(d, CG, "double", 1e-11, 60000) for d in dirs])
+    for i, r in enumerate(conv):
+        assert r.status == 0, (i, items[i].mask.shape, r.iterations)
+    yield items, vols, dirs, conv
+    for v in vols:
+        v.close()
+
+
+def _errs(out, conv):
+    return np.array([abs(r.tau - c.tau) / c.tau for r, c in zip(out, conv)])
+
+
+def test_mixed_cg_batch_within_1e5_of_converged(ctx, batch96):
+    """The plate-like volumes (large r) are ill-conditioned: a relative residual of 1e-7 leaves
+    tau up to ~1e-4 off there, so the batch is solved to 1e-8."""
+    items, vols, dirs, conv = batch96
+    print("mixed CG batch: solving", flush=True)
+    out = T.solve_batch(ctx, vols, [T.make_params(d, CG, "mixed", 1e-8, 60000) for d in dirs])
+    for i, r in enumerate(out):
+        assert r.status == 0, (i, items[i].mask.shape, items[i].r, r.iterations)
+    errs = _errs(out, conv)
+    worst = int(np.argmax(errs))
+    print(f"mixed CG: max rel err {errs.max():.2e} (volume {worst}, shape {items[worst].mask.shape}, "
+          f"r={items[worst].r:.2f}), median {np.median(errs):.2e}, iterations "
+          f"{min(r.iterations for r in out)}..{max(r.iterations for r in out)}")
+    assert errs.max() <= 1e-5
+
+
+def test_mixed_bicgstab_batch_as_accurate_as_fp64(ctx, batch96):
+    """Mixed BiCGSTAB against FP64 BiCGSTAB at the same tolerance (1e-7): every volume
+    converges and its tau is within max(1e-5, 3x the FP64 BiCGSTAB error) of the converged tau."""
+    items, vols, dirs, conv = batch96
+    print("FP64 / mixed BiCGSTAB batches: solving", flush=True)
+    ref = T.solve_batch(ctx, vols, [T.make_params(d, BICG, "double", 1e-7, 60000) for d in dirs])
+    out = T.solve_batch(ctx, vols, [T.make_params(d, BICG, "mixed", 1e-7, 60000) for d in dirs])
+    e_ref, e_mix = _errs(ref, conv), _errs(out, conv)
+    print(f"BiCGSTAB 1e-7: fp64 max err {e_ref.max():.2e} median {np.median(e_ref):.2e}; mixed max err "
+          f"{e_mix.max():.2e} median {np.median(e_mix):.2e}; mixed iterations "
+          f"{min(r.iterations for r in out)}..{max(r.iterations for r in out)}")
+    for i, r in enumerate(out):
+        assert r.status == 0, (i, items[i].mask.shape, items[i].r, r.iterations)
+        assert e_mix[i] <= max(1e-5, 3 * e_ref[i]), (i, e_mix[i], e_ref[i])
+
+
+def test_fp64_batch_equals_solo(ctx, batch96):
+    items, vols, dirs, _ = batch96
+    pick = [0, 17, 42, 95]
+    out = T.solve_batch(ctx, [vols[i] for i in pick], [T.make_params(dirs[i], BICG, "double", 1e-6, 20000)
+                                                       for i in pick])
+    for k, i in enumerate(pick):
+        solo, _ = vols[i].solve(T.make_params(dirs[i], BICG, "double", 1e-6, 20000))
+        assert out[k].tau == solo.tau and out[k].iterations == solo.iterations, i

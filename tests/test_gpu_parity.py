@@ -148,21 +148,25 @@ def test_fp32_dropin_matches_float_oracle(ctx):
 @pytest.mark.parametrize("solver", [CG, BICG])
 def test_mixed_matches_converged_oracle(ctx, solver):
     """The mixed form drives the FP64 true residual below its target; its tau then matches
-    the converged FP64 oracle to the accuracy that residual buys."""
+    the converged FP64 oracle to the accuracy that residual buys.  For CG that is the FP64
+    solve's own accuracy at the same tolerance (bars 3e-5 at 1e-6, 5e-7 at 1e-8).  BiCGSTAB's
+    tau at a given tolerance is itself less accurate, so its bar is also allowed to be three
+    times the FP64 oracle's BiCGSTAB error at that tolerance."""
     m = F.generate_spheres(64, 6.0, 0.60, 0)
     vol = ctx.volume(m)
     for d in (0, 2, 4):
         tight, _ = O.tau(m, d, solver=CG, tol=1e-12)
-        ref, _ = O.tau(m, d, solver=solver, tol=1e-6)
         for outer, bar in ((1e-6, 3e-5), (1e-8, 5e-7)):
+            ref, _ = O.tau(m, d, solver=solver, tol=outer)
+            ref_err = abs(ref.tau - tight.tau) / tight.tau
             res, _ = vol.solve(T.make_params(d, solver, "mixed", outer, 10000))
             err = abs(res.tau - tight.tau) / tight.tau
             print(f"mixed solver={solver} d={d} tol={outer:g} tau={res.tau!r} it={res.iterations} "
                   f"updates={res.refinements} true_res={res.true_rel_residual:.3e} err_vs_converged={err:.2e} "
-                  f"(fp64 oracle it={ref.iterations})")
+                  f"(fp64 oracle it={ref.iterations} err={ref_err:.2e})")
             assert res.status == 0
             assert res.true_rel_residual <= outer
-            assert err <= bar
+            assert err <= (bar if solver == CG else max(bar, 3 * ref_err))
 
 
 def test_solve_is_deterministic(ctx):
