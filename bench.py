@@ -126,6 +126,37 @@ def serving(ctx, T, W, n_req: int, loads=(0.5, 0.8, 1.1)) -> dict:
     return out
 
 
+def profile_program(kind: str, n: int) -> int:
+    """Short, traceable runs for rocprofv3 (kernel trace or counters)."""
+    os.environ["FAST_TAU_GRAPHS"] = "0"   # every kernel launched directly
+    from fast_amd import tau as T
+    from fast_amd import workloads as W
+    from fast_amd._lib import lib
+    mask = W.ellipsoids_256(0) if n == 256 else T.generate_ellipsoids(n, 12.0, 0.35, 0.65, 0)
+    ctx = T.Context(0)
+    vol = ctx.volume(mask)
+    if kind == "solves":
+        for name, solver, prec in (("fp64 bicgstab", BICG, "double"), ("fp64 cg", CG, "double"),
+                                   ("mixed cg", CG, "mixed")):
+            res, _ = vol.solve(T.make_params(0, solver, prec, TOL, 10000))
+            print(f"{name}: tau {res.tau!r} iterations {res.iterations} {res.solve_ms:.1f} ms", flush=True)
+    else:
+        ms = (C.c_double * 4)()
+        for name, prec, solver in (("fp64 bicgstab", 0, BICG), ("fp64 cg", 0, CG), ("mixed cg", 2, CG)):
+            lib().fast_debug_time_passes(vol.handle, 0, prec, solver, 20, ms)
+            print(f"{name}: pass ms {list(ms)[:3]} iteration ms {ms[3]}", flush=True)
+        import torch   # calibration kernel of known traffic: read 256 MiB, write 256 MiB
+        a = torch.ones(64 * 2 ** 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print("copy 256 MiB x 3 done", flush=True)
+    vol.close()
+    ctx.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,11 +166,17 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-iters", type=int, default=25)
     ap.add_argument("--serve", type=int, default=120, help="requests per offered load (0: skip serving)")
+    ap.add_argument("--profile", choices=["solves", "passes"],
+                    help="short program for rocprofv3 instead of the benchmark: 'solves' runs the three X_POS "
+                         "solves (FP64 BiCGSTAB, FP64 CG, mixed CG) with direct launches; 'passes' times 20 "
+                         "iterations of each and runs a 256 MiB device copy for counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(independent volumes shard as one queue per GPU with no communication)", file=sys.stderr)
         return 2
+    if args.profile:
+        return profile_program(args.profile, args.size)
 
     import fast_amd as F
     from fast_amd import tau as T

@@ -122,6 +122,7 @@ struct fast_server {
     Coef<double>* d_coef = nullptr;
     VolState<double>* d_st = nullptr;
     unsigned long long* d_zero = nullptr;     // per-slot zero-voxel counters
+    double* d_tab = nullptr;                  // per-slot diagonal tables
     int* d_tile = nullptr;
     double* d_part = nullptr;
     int* d_pvol = nullptr;
@@ -237,6 +238,7 @@ int rebuild(fast_server* s, const std::vector<Req*>& adm) {
     L.coef = s->d_coef;
     L.st = s->d_st;
     L.part = s->d_part;
+    L.tab = s->d_tab;
     float* a = (float*)s->arena_mem;
     const size_t S = s->S;
     L.vs.x = a + 0 * S;
@@ -248,6 +250,10 @@ int rebuild(fast_server* s, const std::vector<Req*>& adm) {
     L.vs.v[1] = a + 6 * S;
     L.vs.s = a + 7 * S;
     L.vs.x64 = (double*)(a + 8 * S);
+    if (!adm.empty()) {
+        const hipError_t et = ft::build_tables(L, ctx->stream);   // descriptors and coefficients are on the device
+        if (!hip_ok(ctx, et, "server tables")) return FAST_E_HIP;
+    }
     return FAST_OK;
 }
 
@@ -442,7 +448,8 @@ int fast_server_create(fast_ctx* ctx, const fast_server_config* cfg, fast_server
     s->d_coef = (Coef<double>*)dalloc(ctx, sizeof(Coef<double>) * s->max_active);
     s->d_st = (VolState<double>*)dalloc(ctx, sizeof(VolState<double>) * s->max_active);
     s->d_zero = (unsigned long long*)dalloc(ctx, sizeof(unsigned long long) * s->max_active);
-    if (!s->arena_mem || !s->d_vol || !s->d_coef || !s->d_st || !s->d_zero ||
+    s->d_tab = (double*)dalloc(ctx, sizeof(double) * 2 * kTabEntries * s->max_active);
+    if (!s->arena_mem || !s->d_vol || !s->d_coef || !s->d_st || !s->d_zero || !s->d_tab ||
         hipHostMalloc((void**)&s->h_st, sizeof(VolState<double>) * s->max_active, hipHostMallocDefault) != hipSuccess ||
         hipStreamCreateWithFlags(&s->up, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&s->up_done) != hipSuccess) {
         (void)hipGetLastError();
@@ -550,6 +557,7 @@ void fast_server_destroy(fast_server* s) {
     dfree(ctx, s->d_coef);
     dfree(ctx, s->d_st);
     dfree(ctx, s->d_zero);
+    dfree(ctx, s->d_tab);
     dfree(ctx, s->d_tile);
     dfree(ctx, s->d_part);
     dfree(ctx, s->d_pvol);

@@ -535,6 +535,8 @@ static int prepare(Batch<T, A>& B) {
     o = align_up(o + sizeof(int) * planes, 256);
     const size_t o_pout = o;
     o = align_up(o + sizeof(double) * planes, 256);
+    const size_t o_tab = o;
+    o = align_up(o + sizeof(typename Batch<T, A>::C) * 2 * kTabEntries * nv, 256);
     const size_t meta_bytes = o;
     B.meta = dalloc(ctx, meta_bytes);
     const int nvec = 8;   // x, r, r0, p[2], v[2], s
@@ -566,6 +568,7 @@ static int prepare(Batch<T, A>& B) {
     B.plane_vol = (int*)(m + o_pvol);
     B.plane_k = (int*)(m + o_pk);
     B.plane_out = (double*)(m + o_pout);
+    B.L.tab = (typename Batch<T, A>::C*)(m + o_tab);
     T* vb = (T*)B.vecs;
     VecSet& vs = B.L.vs;
     vs.x = vb + 0 * total;
@@ -577,7 +580,8 @@ static int prepare(Batch<T, A>& B) {
     vs.v[1] = vb + 6 * total;
     vs.s = vb + 7 * total;
     vs.x64 = B.mixed ? (double*)(vb + 8 * total) : nullptr;
-    return FAST_OK;
+    e = ft::build_tables(B.L, ctx->stream);
+    return hip_ok(ctx, e, "diagonal tables") ? FAST_OK : FAST_E_HIP;
 }
 
 // Poll the device-resident states every `chunk` iterations, keeping one chunk in flight
@@ -939,7 +943,7 @@ static int time_passes(fast_volume* vol, int dir, int solver, int iters, double*
     if (rc) return rc;
     hipStream_t s = ctx->stream;
     const bool cg = (solver == FAST_CG);
-    const int nev = cg ? 4 : 6;
+    const int nev = cg ? 4 : 6;   // events around each stencil kernel
     std::vector<hipEvent_t> ev((size_t)iters * nev);
     for (auto& e : ev) (void)hipEventCreate(&e);
     hipError_t e = cg ? Passes<T, A>::init_cg(B.L, s) : Passes<T, A>::init_bicg(B.L, s);

@@ -93,10 +93,20 @@ struct Launch {
     const Coef<C>* coef;      // device: per volume, in the arithmetic type
     VolState<A>* st;          // device: per volume
     double* part;             // device: ntiles * 8 partial sums ((hi, lo) pairs)
+    C* tab;                   // device: per volume, kTabEntries diagonals then as many inverses
     VecSet vs;
 };
 
+// Diagonal tables: entry (code | box_x << 7) holds the diagonal of a row with stencil code
+// `code` whose only box faces are the x faces in box_x (bit 0: x = nx-1, bit 1: x = 0);
+// entry kTabEntries + i holds 1 / entry i.  Rows on a y or z box face compute theirs.
+constexpr int kTabEntries = 512;
+
 // ---- launchers (all asynchronous on the given stream) ----
+// Diagonal tables of every volume of a launch (L.tab), from L.coef and the volumes' primary axes.
+template <class T, class A>
+hipError_t build_tables(const Launch<T, A>& L, hipStream_t s);
+
 hipError_t build_code(const uint8_t* mask, uint8_t* code, int nx, int ny, int nz,
                       unsigned long long* zero_count, hipStream_t s);
 
@@ -108,14 +118,13 @@ struct Passes {
     static hipError_t init_cg(const Launch<T, A>&, hipStream_t);
     // BiCGSTAB restart of volumes in ST_RESTART: r = b - A x, r0 = r.
     static hipError_t restart(const Launch<T, A>&, hipStream_t);
-    // One BiCGSTAB iteration (3 fused passes + 3 per-volume scalar updates).
+    // One BiCGSTAB iteration: 3 stencil passes, each followed by its per-volume finalize.
     static hipError_t bicg_iter(const Launch<T, A>&, hipStream_t);
-    // Same, recording ev[2k] / ev[2k+1] around pass k (k = 0, 1, 2) for timing.
+    // Same, recording ev[2k] / ev[2k+1] around stencil pass k (k = 0, 1, 2).
     static hipError_t bicg_iter_timed(const Launch<T, A>&, hipStream_t, hipEvent_t* ev);
-    // One PCG iteration (2 fused passes + 2 scalar updates).
+    // One PCG iteration: 2 stencil passes, each followed by its per-volume finalize.
     static hipError_t cg_iter(const Launch<T, A>&, hipStream_t);
-    // Same, recording ev[0] / ev[1] around the stencil pass and ev[2] / ev[3] around the
-    // elementwise pass.
+    // Same, recording ev[2k] / ev[2k+1] around stencil pass k (k = 0, 1).
     static hipError_t cg_iter_timed(const Launch<T, A>&, hipStream_t, hipEvent_t* ev);
     // y = A x (diagnostic; same device code as the solver).
     static hipError_t apply(const Launch<T, A>&, const T* x, T* y, hipStream_t);

@@ -35,7 +35,8 @@ namespace ft {
 
 namespace {
 
-constexpr int kPart = 8;   // partial-sum doubles per tile: up to 3 sums as (hi, lo) pairs
+constexpr int kPart = 8;         // partial-sum doubles per tile: up to 4 sums as (hi, lo) pairs
+constexpr int kFinThreads = 256; // threads of a per-volume finalize block
 
 enum PassKind : int {
     K_INIT_BICG = 0,
@@ -108,13 +109,6 @@ __device__ __forceinline__ T face_val(const Coef<T>& c, int own, int nbr) {
     return pick2(s == 2, pick2(s == 1, c.ci0, c.ci1), c.ci2);
 }
 
-// off-diagonal coefficient of face j (0 on a box face)
-template <class T>
-__device__ __forceinline__ T off_coef(const Coef<T>& c, int code, int bx, int j) {
-    const T v = face_val(c, (code >> 6) & 1, (code >> j) & 1);
-    return pick2(((bx >> j) & 1) != 0, v, T(0));
-}
-
 // coefficient of face j as it enters the diagonal: interior face, half cell on a primary
 // box face, nothing on a lateral box face (Diffusion.cuh:93-105)
 template <class T>
@@ -152,18 +146,41 @@ __device__ __forceinline__ T x0_of(int k, int np, int sgn) {
     return T(k) / T(np + 1);
 }
 
-// Row product in CSR column order Z-,Y-,X-,diag,X+,Y+,Z+.
+// Diagonal and inverse diagonal of a row from the volume's tables (in LDS).  Rows on a y or z
+// box face evaluate the formula: they fill whole rows or planes of the march, so for 64-wide
+// tiles the branch is uniform over a wave.  The tables were filled by diag_of itself, so both
+// ways give the same bits.
+template <class C>
+__device__ __forceinline__ C diag_at(const C* tab, int code, int bx, int primary, const Coef<C>& c) {
+    if (bx & 0x3C) return diag_of<C>(code, bx, primary, c);
+    return tab[code | ((bx & 3) << 7)];
+}
+
+template <class C>
+__device__ __forceinline__ C inv_at(const C* tab, int code, int bx, int primary, const Coef<C>& c) {
+    if (bx & 0x3C) return C(1) / diag_of<C>(code, bx, primary, c);
+    return tab[kTabEntries + (code | ((bx & 3) << 7))];
+}
+
+// Row product in CSR column order Z-,Y-,X-,diag,X+,Y+,Z+.  A box face has no off-diagonal
+// coefficient in the reference; here its term is c * (+0): the plane march supplies +0 for
+// every neighbour outside the box, c is finite and positive, and the accumulator (started
+// at +0) is never -0, so the term leaves the sum unchanged bit for bit.
 template <class T>
-__device__ __forceinline__ T apply_row(int code, int bx, T dg, const Coef<T>& c, T uzm, T uym, T uxm, T uc,
-                                       T uxp, T uyp, T uzp) {
+__device__ __forceinline__ T apply_row(int code, T dg, const Coef<T>& c, T uzm, T uym, T uxm, T uc, T uxp, T uyp,
+                                       T uzp) {
+    // face coefficient ci[own + nbr]: choose the pair by the own flag once, then per face
+    const bool own = ((code >> 6) & 1) != 0;
+    const T lo = pick2(own, c.ci0, c.ci1);   // neighbour pore
+    const T hi = pick2(own, c.ci1, c.ci2);   // neighbour solid
     T acc = T(0);
-    acc += off_coef(c, code, bx, 5) * uzm;
-    acc += off_coef(c, code, bx, 3) * uym;
-    acc += off_coef(c, code, bx, 1) * uxm;
+    acc += pick2(((code >> 5) & 1) != 0, lo, hi) * uzm;
+    acc += pick2(((code >> 3) & 1) != 0, lo, hi) * uym;
+    acc += pick2(((code >> 1) & 1) != 0, lo, hi) * uxm;
     acc += dg * uc;
-    acc += off_coef(c, code, bx, 0) * uxp;
-    acc += off_coef(c, code, bx, 2) * uyp;
-    acc += off_coef(c, code, bx, 4) * uzp;
+    acc += pick2((code & 1) != 0, lo, hi) * uxp;
+    acc += pick2(((code >> 2) & 1) != 0, lo, hi) * uyp;
+    acc += pick2(((code >> 4) & 1) != 0, lo, hi) * uzp;
     return acc;
 }
 
@@ -244,6 +261,13 @@ __device__ __forceinline__ void block_reduce_store(Acc (&acc)[NP], double* out) 
 // (and the centre values the epilogue needs).  epi(): outputs and partial sums at the centre.
 // fetch and compute are split so the plane march can issue loads two planes ahead.
 // ---------------------------------------------------------------------------------------
+// Load through a global-address-space pointer (a flat load would also count against the LDS
+// counter and be waited for at every barrier of the plane march).
+template <class T>
+__device__ __forceinline__ T gl(const T* p, long long i) {
+    return ((const __attribute__((address_space(1))) T*)p)[i];
+}
+
 template <class T>
 struct Raw {
     T a, b, c, e, f, g;
@@ -252,7 +276,7 @@ struct Raw {
 
 template <class T>
 struct Pt {
-    T u, aux, e, f, g, dg;
+    T u, aux, e, f, g, dg, iv;   // iv = 1 / dg where a pass needs it
     int code;
 };
 
@@ -264,6 +288,7 @@ template <class T, class A>
 struct PassCtx {
     using C = typename CmpOf<T, A>::type;
     Coef<C> c;
+    const C* tab;   // diagonal tables of the volume (LDS)
     int primary, sgn, np, first;
     C alpha, beta, omega;
     const uint8_t* code;
@@ -284,19 +309,20 @@ struct PassOps;
 template <class T, class A>
 struct PassOps<T, A, K_INIT_BICG> {
     using C = typename CmpOf<T, A>::type;
+    using R = Raw<T>;   // raw loads in the storage type; widened in compute()
     static constexpr int NP = 2;   // |r|^2, |b|^2
     template <class X>
-    __device__ static Raw<C> fetch(const X& pc, long long idx, long long lin, bool centre) {
-        Raw<C> w;
-        w.code = pc.code[lin];
+    __device__ static R fetch(const X& pc, long long idx, long long lin, bool centre) {
+        R w;
+        w.code = gl(pc.code, lin);
         return w;
     }
     template <class X>
-    __device__ static Pt<C> compute(const X& pc, const Raw<C>& w, int bx, int k) {
+    __device__ static Pt<C> compute(const X& pc, const R& w, int bx, int k) {
         Pt<C> p;
         p.code = w.code;
         p.u = x0_of<C>(k, pc.np, pc.sgn);
-        p.dg = diag_of<C>(w.code, bx, pc.primary, pc.c);
+        p.dg = diag_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
         return p;
     }
     template <class X, class Acc>
@@ -317,26 +343,28 @@ struct PassOps<T, A, K_INIT_BICG> {
 template <class T, class A>
 struct PassOps<T, A, K_INIT_CG> {
     using C = typename CmpOf<T, A>::type;
+    using R = Raw<T>;   // raw loads in the storage type; widened in compute()
     static constexpr int NP = 3;   // |r|^2, |b|^2, r.(M^-1 r)
     template <class X>
-    __device__ static Raw<C> fetch(const X& pc, long long idx, long long lin, bool centre) {
-        Raw<C> w;
-        w.code = pc.code[lin];
+    __device__ static R fetch(const X& pc, long long idx, long long lin, bool centre) {
+        R w;
+        w.code = gl(pc.code, lin);
         return w;
     }
     template <class X>
-    __device__ static Pt<C> compute(const X& pc, const Raw<C>& w, int bx, int k) {
+    __device__ static Pt<C> compute(const X& pc, const R& w, int bx, int k) {
         Pt<C> p;
         p.code = w.code;
         p.u = x0_of<C>(k, pc.np, pc.sgn);
-        p.dg = diag_of<C>(w.code, bx, pc.primary, pc.c);
+        p.dg = diag_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
+        p.iv = inv_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
         return p;
     }
     template <class X, class Acc>
     __device__ static void epi(const X& pc, long long idx, int k, C Au, const Pt<C>& pt, Acc (&acc)[NP]) {
         const C b = rhs_of<C>(pt.code, k, pc.np, pc.c);
         const C r = b - Au;
-        const C inv = C(1) / pt.dg;
+        const C inv = pt.iv;
         pc.x[idx] = T(pt.u);
         pc.r[idx] = T(r);
         acc[0].add(r, r);
@@ -349,20 +377,21 @@ struct PassOps<T, A, K_INIT_CG> {
 template <class T, class A>
 struct PassOps<T, A, K_RESTART> {
     using C = typename CmpOf<T, A>::type;
+    using R = Raw<T>;   // raw loads in the storage type; widened in compute()
     static constexpr int NP = 1;
     template <class X>
-    __device__ static Raw<C> fetch(const X& pc, long long idx, long long lin, bool centre) {
-        Raw<C> w;
-        w.a = C(pc.x[idx]);
-        w.code = pc.code[lin];
+    __device__ static R fetch(const X& pc, long long idx, long long lin, bool centre) {
+        R w;
+        w.a = gl(pc.x, idx);
+        w.code = gl(pc.code, lin);
         return w;
     }
     template <class X>
-    __device__ static Pt<C> compute(const X& pc, const Raw<C>& w, int bx, int k) {
+    __device__ static Pt<C> compute(const X& pc, const R& w, int bx, int k) {
         Pt<C> p;
         p.code = w.code;
-        p.u = w.a;
-        p.dg = diag_of<C>(w.code, bx, pc.primary, pc.c);
+        p.u = C(w.a);
+        p.dg = diag_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
         return p;
     }
     template <class X, class Acc>
@@ -379,20 +408,21 @@ struct PassOps<T, A, K_RESTART> {
 template <class T, class A>
 struct PassOps<T, A, K_APPLY> {
     using C = typename CmpOf<T, A>::type;
+    using R = Raw<T>;   // raw loads in the storage type; widened in compute()
     static constexpr int NP = 0;
     template <class X>
-    __device__ static Raw<C> fetch(const X& pc, long long idx, long long lin, bool centre) {
-        Raw<C> w;
-        w.a = C(pc.x[idx]);
-        w.code = pc.code[lin];
+    __device__ static R fetch(const X& pc, long long idx, long long lin, bool centre) {
+        R w;
+        w.a = gl(pc.x, idx);
+        w.code = gl(pc.code, lin);
         return w;
     }
     template <class X>
-    __device__ static Pt<C> compute(const X& pc, const Raw<C>& w, int bx, int k) {
+    __device__ static Pt<C> compute(const X& pc, const R& w, int bx, int k) {
         Pt<C> p;
         p.code = w.code;
-        p.u = w.a;
-        p.dg = diag_of<C>(w.code, bx, pc.primary, pc.c);
+        p.u = C(w.a);
+        p.dg = diag_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
         return p;
     }
     template <class X, class Acc>
@@ -405,27 +435,29 @@ struct PassOps<T, A, K_APPLY> {
 template <class T, class A>
 struct PassOps<T, A, K_BICG1> {
     using C = typename CmpOf<T, A>::type;
+    using R = Raw<T>;   // raw loads in the storage type; widened in compute()
     static constexpr int NP = 1;
     template <class X>
-    __device__ static Raw<C> fetch(const X& pc, long long idx, long long lin, bool centre) {
-        Raw<C> w;
-        w.a = C(pc.r[idx]);
-        w.b = C(pc.p_old[idx]);
-        w.c = C(pc.v_old[idx]);
-        if (centre) w.e = C(pc.r0[idx]);
-        w.code = pc.code[lin];
+    __device__ static R fetch(const X& pc, long long idx, long long lin, bool centre) {
+        R w;
+        w.a = gl(pc.r, idx);
+        w.b = gl(pc.p_old, idx);
+        w.c = gl(pc.v_old, idx);
+        if (centre) w.e = gl(pc.r0, idx);
+        w.code = gl(pc.code, lin);
         return w;
     }
     template <class X>
-    __device__ static Pt<C> compute(const X& pc, const Raw<C>& w, int bx, int k) {
+    __device__ static Pt<C> compute(const X& pc, const R& w, int bx, int k) {
         Pt<C> q;
         q.code = w.code;
         // p as it will be stored: the stencil and the x update must see the same direction
-        const C p = C(T(w.a + pc.beta * (w.b - pc.omega * w.c)));
-        q.dg = diag_of<C>(w.code, bx, pc.primary, pc.c);
-        q.u = (C(1) / q.dg) * p;
+        const C p = C(T(C(w.a) + pc.beta * (C(w.b) - pc.omega * C(w.c))));
+        q.dg = diag_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
+        q.iv = inv_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
+        q.u = q.iv * p;
         q.aux = p;
-        q.e = w.e;
+        q.e = C(w.e);
         return q;
     }
     template <class X, class Acc>
@@ -442,22 +474,24 @@ struct PassOps<T, A, K_BICG1> {
 template <class T, class A>
 struct PassOps<T, A, K_BICG2> {
     using C = typename CmpOf<T, A>::type;
+    using R = Raw<T>;   // raw loads in the storage type; widened in compute()
     static constexpr int NP = 2;
     template <class X>
-    __device__ static Raw<C> fetch(const X& pc, long long idx, long long lin, bool centre) {
-        Raw<C> w;
-        w.a = C(pc.r[idx]);
-        w.b = C(pc.v_new[idx]);
-        w.code = pc.code[lin];
+    __device__ static R fetch(const X& pc, long long idx, long long lin, bool centre) {
+        R w;
+        w.a = gl(pc.r, idx);
+        w.b = gl(pc.v_new, idx);
+        w.code = gl(pc.code, lin);
         return w;
     }
     template <class X>
-    __device__ static Pt<C> compute(const X& pc, const Raw<C>& w, int bx, int k) {
+    __device__ static Pt<C> compute(const X& pc, const R& w, int bx, int k) {
         Pt<C> q;
         q.code = w.code;
-        const C s = C(T(w.a - pc.alpha * w.b));
-        q.dg = diag_of<C>(w.code, bx, pc.primary, pc.c);
-        q.u = (C(1) / q.dg) * s;
+        const C s = C(T(C(w.a) - pc.alpha * C(w.b)));
+        q.dg = diag_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
+        q.iv = inv_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
+        q.u = q.iv * s;
         q.aux = s;
         return q;
     }
@@ -475,35 +509,36 @@ struct PassOps<T, A, K_BICG2> {
 template <class T, class A>
 struct PassOps<T, A, K_BICG3> {
     using C = typename CmpOf<T, A>::type;
+    using R = Raw<T>;   // raw loads in the storage type; widened in compute()
     static constexpr int NP = 2;
     template <class X>
-    __device__ static Raw<C> fetch(const X& pc, long long idx, long long lin, bool centre) {
-        Raw<C> w;
-        w.a = C(pc.s[idx]);
+    __device__ static R fetch(const X& pc, long long idx, long long lin, bool centre) {
+        R w;
+        w.a = gl(pc.s, idx);
         if (centre) {
-            w.c = C(pc.p_new[idx]);
-            w.e = C(pc.r0[idx]);
-            w.f = C(pc.x[idx]);
+            w.c = gl(pc.p_new, idx);
+            w.e = gl(pc.r0, idx);
+            w.f = gl(pc.x, idx);
         }
-        w.code = pc.code[lin];
+        w.code = gl(pc.code, lin);
         return w;
     }
     template <class X>
-    __device__ static Pt<C> compute(const X& pc, const Raw<C>& w, int bx, int k) {
+    __device__ static Pt<C> compute(const X& pc, const R& w, int bx, int k) {
         Pt<C> q;
         q.code = w.code;
-        q.dg = diag_of<C>(w.code, bx, pc.primary, pc.c);
-        q.u = (C(1) / q.dg) * w.a;
-        q.aux = w.a;
-        q.e = w.e;
-        q.f = w.f;
-        q.g = w.c;
+        q.dg = diag_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
+        q.iv = inv_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
+        q.u = q.iv * C(w.a);
+        q.aux = C(w.a);
+        q.e = C(w.e);
+        q.f = C(w.f);
+        q.g = C(w.c);
         return q;
     }
     template <class X, class Acc>
     __device__ static void epi(const X& pc, long long idx, int k, C Au, const Pt<C>& pt, Acc (&acc)[NP]) {
-        const C inv = C(1) / pt.dg;
-        const C yv = inv * pt.g;
+        const C yv = pt.iv * pt.g;
         pc.x[idx] = T(pt.f + (pc.alpha * yv + pc.omega * pt.u));
         const T rs = T(pt.aux - pc.omega * Au);   // the stored residual is what the next iteration sees
         pc.r[idx] = rs;
@@ -518,23 +553,25 @@ struct PassOps<T, A, K_BICG3> {
 template <class T, class A>
 struct PassOps<T, A, K_CG1> {
     using C = typename CmpOf<T, A>::type;
+    using R = Raw<T>;   // raw loads in the storage type; widened in compute()
     static constexpr int NP = 1;
     template <class X>
-    __device__ static Raw<C> fetch(const X& pc, long long idx, long long lin, bool centre) {
-        Raw<C> w;
-        w.a = C(pc.r[idx]);
-        if (!pc.first) w.b = C(pc.p_old[idx]);
-        w.code = pc.code[lin];
+    __device__ static R fetch(const X& pc, long long idx, long long lin, bool centre) {
+        R w;
+        w.a = gl(pc.r, idx);
+        w.b = gl(pc.p_old, idx);   // unused on the first iteration (an unconditional load stays in flight)
+        w.code = gl(pc.code, lin);
         return w;
     }
     template <class X>
-    __device__ static Pt<C> compute(const X& pc, const Raw<C>& w, int bx, int k) {
+    __device__ static Pt<C> compute(const X& pc, const R& w, int bx, int k) {
         Pt<C> q;
         q.code = w.code;
-        q.dg = diag_of<C>(w.code, bx, pc.primary, pc.c);
-        const C z = (C(1) / q.dg) * w.a;
+        q.dg = diag_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
+        q.iv = inv_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
+        const C z = q.iv * C(w.a);
         // p as it will be stored: the stencil and the x update must see the same direction
-        q.u = C(T(pc.first ? z : z + pc.beta * w.b));
+        q.u = C(T(pc.first ? z : z + pc.beta * C(w.b)));
         return q;
     }
     template <class X, class Acc>
@@ -549,26 +586,28 @@ struct PassOps<T, A, K_CG1> {
 template <class T, class A>
 struct PassOps<T, A, K_CG2> {
     using C = typename CmpOf<T, A>::type;
+    using R = Raw<T>;   // raw loads in the storage type; widened in compute()
     static constexpr int NP = 2;
     template <class X>
-    __device__ static Raw<C> fetch(const X& pc, long long idx, long long lin, bool centre) {
-        Raw<C> w;
-        w.a = C(pc.p_new[idx]);
+    __device__ static R fetch(const X& pc, long long idx, long long lin, bool centre) {
+        R w;
+        w.a = gl(pc.p_new, idx);
         if (centre) {
-            w.b = C(pc.x[idx]);
-            w.c = C(pc.r[idx]);
+            w.b = gl(pc.x, idx);
+            w.c = gl(pc.r, idx);
         }
-        w.code = pc.code[lin];
+        w.code = gl(pc.code, lin);
         return w;
     }
     template <class X>
-    __device__ static Pt<C> compute(const X& pc, const Raw<C>& w, int bx, int k) {
+    __device__ static Pt<C> compute(const X& pc, const R& w, int bx, int k) {
         Pt<C> q;
         q.code = w.code;
-        q.dg = diag_of<C>(w.code, bx, pc.primary, pc.c);
-        q.u = w.a;
-        q.aux = w.b;
-        q.e = w.c;
+        q.dg = diag_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
+        q.iv = inv_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
+        q.u = C(w.a);
+        q.aux = C(w.b);
+        q.e = C(w.c);
         return q;
     }
     template <class X, class Acc>
@@ -577,7 +616,7 @@ struct PassOps<T, A, K_CG2> {
         const T rs = T(pt.e - pc.alpha * Au);   // the stored residual is what the next iteration sees
         pc.r[idx] = rs;
         acc[0].add(C(rs), C(rs));
-        acc[1].add(C(rs), (C(1) / pt.dg) * C(rs));
+        acc[1].add(C(rs), pt.iv * C(rs));
     }
 };
 
@@ -589,8 +628,10 @@ struct PassOps<T, A, K_CG2> {
 // ---------------------------------------------------------------------------------------
 template <class U, class Ops, class Ctx, class Acc, int NPA, int TX, int TY>
 __device__ __forceinline__ void march(const Ctx& pc, const VolDesc& vd, int lt, U* lds, Acc (&acc)[NPA]) {
+    using R = typename Ops::R;
     constexpr int W = TX + 2;
     constexpr int PL = (TY + 2) * W;
+    constexpr int HWAVES = (2 * TX + 2 * TY + 63) / 64;   // leading waves that stage halo points
     const int nx = vd.nx, ny = vd.ny, nz = vd.nz, primary = vd.primary;
     const int tix = lt % vd.tiles_x;
     const int tiy = (lt / vd.tiles_x) % vd.tiles_y;
@@ -622,59 +663,66 @@ __device__ __forceinline__ void march(const Ctx& pc, const VolDesc& vd, int lt, 
         hl = (h - 2 * TX - TY + 1) * W + (TX + 1);
     }
     const bool hvalid = (hl >= 0) && hx >= 0 && hy >= 0 && hx < nx && hy < ny;
+    const bool hwave = (threadIdx.x >> 6) < HWAVES;   // uniform over the wave
 
+    // Every lane loads a voxel inside the volume (clamped coordinates, clamped plane), so the
+    // loads carry no per-lane condition and stay in flight across the plane loop; what a lane
+    // outside the volume (or past the last plane) loads is never used.
+    const int xc = min(x, nx - 1), yc = min(y, ny - 1);
+    const int hxc = min(max(hx, 0), nx - 1), hyc = min(max(hy, 0), ny - 1);
     const long long plane = (long long)nx * ny;
-    const long long clin = (long long)x + (long long)nx * y;    // centre column
-    const long long hlin = (long long)hx + (long long)nx * hy;  // halo column
+    const long long clin = (long long)xc + (long long)nx * yc;    // centre column
+    const long long hlin = (long long)hxc + (long long)nx * hyc;  // halo column
     const long long base = vd.off;
     auto bxz = [&](int bxy, int pz) { return bxy | (pz == nz - 1 ? 16 : 0) | (pz == 0 ? 32 : 0); };
     const int cbxy = (x == nx - 1 ? 1 : 0) | (x == 0 ? 2 : 0) | (y == ny - 1 ? 4 : 0) | (y == 0 ? 8 : 0);
     const int hbxy = (hx == nx - 1 ? 1 : 0) | (hx == 0 ? 2 : 0) | (hy == ny - 1 ? 4 : 0) | (hy == 0 ? 8 : 0);
     auto kc = [&](int px, int py, int pz) { return primary == 0 ? px : (primary == 1 ? py : pz); };
+    auto clampz = [&](int pz) { return pz < nz ? pz : nz - 1; };
 
-    U u_prev = U(0), h_cur = U(0), h_next = U(0);
-    Pt<U> c_cur{}, c_next{};
-    Raw<U> rc{}, rh{};
-    if (inside) {
-        if (z0 > 0) {
-            const long long l = clin + plane * (z0 - 1);
-            u_prev = Ops::compute(pc, Ops::fetch(pc, base + l, l, false), bxz(cbxy, z0 - 1), kc(x, y, z0 - 1)).u;
-        }
+    U u_prev = U(0), h_cur = U(0);
+    Pt<U> c_cur{};
+    R rcA{}, rcB{}, rhA{}, rhB{};
+    if (z0 > 0) {
+        const long long l = clin + plane * (z0 - 1);
+        u_prev = Ops::compute(pc, Ops::fetch(pc, base + l, l, false), bxz(cbxy, z0 - 1), kc(x, y, z0 - 1)).u;
+    }
+    {
         const long long l0 = clin + plane * z0;
         c_cur = Ops::compute(pc, Ops::fetch(pc, base + l0, l0, true), bxz(cbxy, z0), kc(x, y, z0));
-        if (z0 + 1 < nz) {
-            const long long l1 = l0 + plane;
-            rc = Ops::fetch(pc, base + l1, l1, true);
-        }
+        const long long l1 = clin + plane * clampz(z0 + 1);
+        rcA = Ops::fetch(pc, base + l1, l1, true);
     }
-    if (hvalid) {
+    if (hwave) {
         const long long l0 = hlin + plane * z0;
         h_cur = Ops::compute(pc, Ops::fetch(pc, base + l0, l0, false), bxz(hbxy, z0), kc(hx, hy, z0)).u;
-        if (z0 + 1 < z1) rh = Ops::fetch(pc, base + l0 + plane, l0 + plane, false);
+        const long long l1 = hlin + plane * clampz(z0 + 1);
+        rhA = Ops::fetch(pc, base + l1, l1, false);
     }
 
-    for (int z = z0; z < z1; ++z) {
+    // One plane: `rc_in` / `rh_in` hold the loads of plane z+1 (issued one plane ago), the
+    // loads of plane z+2 go to `rc_out` / `rh_out`.  The loop below alternates the two
+    // register sets, so a load in flight is never copied (a copy would have to wait for it).
+    auto step = [&](int z, const R& rc_in, R& rc_out, const R& rh_in, R& rh_out) {
         const bool has_next = (z + 1 < nz);
-        // stencil inputs of plane z+1 from the loads issued one plane ago
-        if (inside && has_next) c_next = Ops::compute(pc, rc, bxz(cbxy, z + 1), kc(x, y, z + 1));
-        if (hvalid && z + 1 < z1) h_next = Ops::compute(pc, rh, bxz(hbxy, z + 1), kc(hx, hy, z + 1)).u;
-        // loads of plane z+2
-        if (inside && z + 2 < nz && z + 1 < z1) {
-            const long long l = clin + plane * (z + 2);
-            rc = Ops::fetch(pc, base + l, l, true);
+        const Pt<U> c_next = Ops::compute(pc, rc_in, bxz(cbxy, z + 1), kc(x, y, z + 1));
+        U h_next = U(0);
+        if (hwave) h_next = Ops::compute(pc, rh_in, bxz(hbxy, z + 1), kc(hx, hy, z + 1)).u;
+        {
+            const long long l = clin + plane * clampz(z + 2);
+            rc_out = Ops::fetch(pc, base + l, l, true);
         }
-        if (hvalid && z + 2 < z1) {
-            const long long l = hlin + plane * (z + 2);
-            rh = Ops::fetch(pc, base + l, l, false);
+        if (hwave) {
+            const long long l = hlin + plane * clampz(z + 2);
+            rh_out = Ops::fetch(pc, base + l, l, false);
         }
         U* buf = lds + (z & 1) * PL;
-        buf[(ty + 1) * W + (tx + 1)] = c_cur.u;
+        buf[(ty + 1) * W + (tx + 1)] = inside ? c_cur.u : U(0);
         if (hl >= 0) buf[hl] = hvalid ? h_cur : U(0);
         __syncthreads();
         if (inside) {
-            const int bx = bxz(cbxy, z);
             const U uzp = has_next ? c_next.u : U(0);
-            const U Au = apply_row<U>(c_cur.code, bx, c_cur.dg, pc.c, u_prev, buf[ty * W + (tx + 1)],
+            const U Au = apply_row<U>(c_cur.code, c_cur.dg, pc.c, u_prev, buf[ty * W + (tx + 1)],
                                       buf[(ty + 1) * W + tx], c_cur.u, buf[(ty + 1) * W + (tx + 2)],
                                       buf[(ty + 2) * W + (tx + 1)], uzp);
             Ops::epi(pc, base + clin + plane * z, kc(x, y, z), Au, c_cur, acc);
@@ -682,7 +730,53 @@ __device__ __forceinline__ void march(const Ctx& pc, const VolDesc& vd, int lt, 
         u_prev = c_cur.u;
         c_cur = c_next;
         h_cur = h_next;
+    };
+    for (int z = z0; z < z1; z += 2) {
+        step(z, rcA, rcB, rhA, rhB);
+        if (z + 1 < z1) step(z + 1, rcB, rcA, rhB, rhA);
     }
+}
+
+// ---------------------------------------------------------------------------------------
+// End of a pass.  Every tile stores its partial sums; then one block per volume (k_fin)
+// reduces that volume's partials in a fixed order and runs its scalar update.  (Finishing in
+// the last tile of each volume instead would need a device-scope release fence in every
+// tile, which writes back the XCD's L2 each time and costs more than the extra launch.)
+// ---------------------------------------------------------------------------------------
+// Sum of the partials of tiles [base, base + n) in a fixed order; thread 0 gets the totals.
+template <int NP, int NT>
+__device__ __forceinline__ void reduce_volume(const double* part, int base, int n, double (&tot)[NP]) {
+    __shared__ double red[2 * (NP > 0 ? NP : 1)];
+    AccDD sums[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) sums[k].init();
+    constexpr int U = 4;   // tiles in flight per thread
+    for (int t0 = threadIdx.x; t0 < n; t0 += U * NT) {
+        double v[U][2 * NP];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int t = t0 + u * NT;
+            const double* p = part + (long long)(base + (t < n ? t : 0)) * kPart;
+#pragma unroll
+            for (int k = 0; k < 2 * NP; ++k) v[u][k] = (t < n) ? p[k] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int k = 0; k < NP; ++k) sums[k].add2(v[u][2 * k], v[u][2 * k + 1]);
+    }
+    block_reduce_store<AccDD, NP, NT>(sums, red);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NP; ++k) tot[k] = red[2 * k] + red[2 * k + 1];
+    }
+}
+
+// Copy the volume's diagonal tables into LDS.
+template <class C, int NT>
+__device__ __forceinline__ void load_tables(const C* src, C* dst) {
+    for (int i = threadIdx.x; i < 2 * kTabEntries; i += NT) dst[i] = src[i];
+    __syncthreads();
 }
 
 template <class T, class A, int TX, int TY, int KIND>
@@ -692,6 +786,7 @@ __global__ __launch_bounds__(TX* TY) void k_stencil(Launch<T, A> L, const T* xin
     constexpr int NPA = NP > 0 ? NP : 1;
     using C = typename CmpOf<T, A>::type;
     __shared__ C lds[2 * (TY + 2) * (TX + 2)];
+    __shared__ C tab[2 * kTabEntries];
 
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
     if (tile >= L.ntiles) return;
@@ -703,6 +798,7 @@ __global__ __launch_bounds__(TX* TY) void k_stencil(Launch<T, A> L, const T* xin
 
     PassCtx<T, A> pc;
     pc.c = L.coef[v];
+    pc.tab = tab;
     pc.primary = vd.primary;
     pc.sgn = vd.sgn;
     pc.np = dim_of(vd, vd.primary);
@@ -726,6 +822,7 @@ __global__ __launch_bounds__(TX* TY) void k_stencil(Launch<T, A> L, const T* xin
             pc.s = yout;
         }
     }
+    load_tables<C, TX * TY>(L.tab + (long long)v * 2 * kTabEntries, tab);
     using Acc = typename AccOf<T, A>::type;
     Acc acc[NPA];
 #pragma unroll
@@ -844,36 +941,10 @@ __device__ __forceinline__ void mixed_cg_after_iteration(VolState<A>* s, A rz) {
     s->status = ST_RUNNING;
 }
 
-template <class T, class A, int KIND>
-__global__ __launch_bounds__(64) void k_fin(Launch<T, A> L) {
-    const int v = blockIdx.x;
-    if (v >= L.nvol) return;
-    VolState<A>* s = L.st + v;
-    const int status = s->status;
-    if (KIND == F_RESTART) {
-        if (status != ST_RESTART) return;
-    } else if (status != ST_RUNNING) {
-        return;
-    }
-    const VolDesc& vd = L.vol[v];
-    const int lane = threadIdx.x;
-    using Acc = typename AccOf<T, A>::type;
-    Acc sums[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) sums[k].init();
-    for (int t = lane; t < vd.ntiles; t += 64) {
-        const double* p = L.part + (long long)(vd.tile_base + t) * kPart;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) sums[k].add2(p[2 * k], p[2 * k + 1]);
-    }
-    A tot[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        wave_combine(sums[k]);
-        tot[k] = A(sums[k].hi + sums[k].lo);
-    }
-    if (lane != 0) return;
-
+// Scalar update of one volume after a pass, from the pass's dot products `tot` (run by one
+// thread of the volume's last tile).
+template <class A, int KIND>
+__device__ void fin_update(VolState<A>* s, const A* tot) {
     switch (KIND) {
         case F_INIT_BICG: {
             s->rsq = tot[0];
@@ -970,35 +1041,65 @@ __global__ __launch_bounds__(64) void k_fin(Launch<T, A> L) {
     }
 }
 
+// partial sums each finalize reads
+__host__ __device__ constexpr int np_of(int fin) {
+    return fin == F_INIT_CG ? 3 : (fin == F_INIT_BICG || fin == F_BICG2 || fin == F_BICG3 || fin == F_CG2) ? 2 : 1;
+}
+
+// One block per volume: reduce the volume's tile partials of the pass just run (fixed order)
+// and update its scalars.
+template <class T, class A, int KIND>
+__global__ __launch_bounds__(kFinThreads) void k_fin(Launch<T, A> L) {
+    constexpr int NP = np_of(KIND);
+    const int v = blockIdx.x;
+    if (v >= L.nvol) return;
+    VolState<A>* s = L.st + v;
+    const int status = s->status;
+    if (KIND == F_RESTART) {
+        if (status != ST_RESTART) return;
+    } else if (status != ST_RUNNING) {
+        return;
+    }
+    const VolDesc& vd = L.vol[v];
+    double tot[NP];
+    reduce_volume<NP, kFinThreads>(L.part, vd.tile_base, vd.ntiles, tot);
+    if (threadIdx.x == 0) {
+        A ta[3] = {A(0), A(0), A(0)};
+#pragma unroll
+        for (int k = 0; k < NP; ++k) ta[k] = A(tot[k]);
+        fin_update<A, KIND>(s, ta);
+    }
+}
+
 template <class T, class A, int TX, int TY>
 hipError_t launch_iter_bicg(const Launch<T, A>& L, hipStream_t s, hipEvent_t* ev = nullptr) {
-    const dim3 g(L.ntiles), b(TX * TY);
+    const dim3 g(L.ntiles), b(TX * TY), gf(L.nvol), bf(kFinThreads);
     if (ev) (void)hipEventRecord(ev[0], s);
     hipLaunchKernelGGL((k_stencil<T, A, TX, TY, K_BICG1>), g, b, 0, s, L, (const T*)nullptr, (T*)nullptr);
     if (ev) (void)hipEventRecord(ev[1], s);
-    hipLaunchKernelGGL((k_fin<T, A, F_BICG1>), dim3(L.nvol), dim3(64), 0, s, L);
+    hipLaunchKernelGGL((k_fin<T, A, F_BICG1>), gf, bf, 0, s, L);
     if (ev) (void)hipEventRecord(ev[2], s);
     hipLaunchKernelGGL((k_stencil<T, A, TX, TY, K_BICG2>), g, b, 0, s, L, (const T*)nullptr, (T*)nullptr);
     if (ev) (void)hipEventRecord(ev[3], s);
-    hipLaunchKernelGGL((k_fin<T, A, F_BICG2>), dim3(L.nvol), dim3(64), 0, s, L);
+    hipLaunchKernelGGL((k_fin<T, A, F_BICG2>), gf, bf, 0, s, L);
     if (ev) (void)hipEventRecord(ev[4], s);
     hipLaunchKernelGGL((k_stencil<T, A, TX, TY, K_BICG3>), g, b, 0, s, L, (const T*)nullptr, (T*)nullptr);
     if (ev) (void)hipEventRecord(ev[5], s);
-    hipLaunchKernelGGL((k_fin<T, A, F_BICG3>), dim3(L.nvol), dim3(64), 0, s, L);
+    hipLaunchKernelGGL((k_fin<T, A, F_BICG3>), gf, bf, 0, s, L);
     return hipGetLastError();
 }
 
 template <class T, class A, int TX, int TY>
 hipError_t launch_iter_cg(const Launch<T, A>& L, hipStream_t s, hipEvent_t* ev = nullptr) {
-    const dim3 g(L.ntiles), b(TX * TY);
+    const dim3 g(L.ntiles), b(TX * TY), gf(L.nvol), bf(kFinThreads);
     if (ev) (void)hipEventRecord(ev[0], s);
     hipLaunchKernelGGL((k_stencil<T, A, TX, TY, K_CG1>), g, b, 0, s, L, (const T*)nullptr, (T*)nullptr);
     if (ev) (void)hipEventRecord(ev[1], s);
-    hipLaunchKernelGGL((k_fin<T, A, F_CG1>), dim3(L.nvol), dim3(64), 0, s, L);
+    hipLaunchKernelGGL((k_fin<T, A, F_CG1>), gf, bf, 0, s, L);
     if (ev) (void)hipEventRecord(ev[2], s);
     hipLaunchKernelGGL((k_stencil<T, A, TX, TY, K_CG2>), g, b, 0, s, L, (const T*)nullptr, (T*)nullptr);
     if (ev) (void)hipEventRecord(ev[3], s);
-    hipLaunchKernelGGL((k_fin<T, A, F_CG2>), dim3(L.nvol), dim3(64), 0, s, L);
+    hipLaunchKernelGGL((k_fin<T, A, F_CG2>), gf, bf, 0, s, L);
     return hipGetLastError();
 }
 
@@ -1041,6 +1142,34 @@ hipError_t build_code(const uint8_t* mask, uint8_t* code, int nx, int ny, int nz
     return hipGetLastError();
 }
 
+// ---- diagonal tables ---------------------------------------------------------------------
+namespace {
+template <class C>
+__global__ __launch_bounds__(256) void k_tables(const VolDesc* vol, const Coef<C>* coef, C* tab, int nvol) {
+    const int v = blockIdx.x;
+    if (v >= nvol) return;
+    const Coef<C> c = coef[v];
+    const int primary = vol[v].primary;
+    C* t = tab + (long long)v * 2 * kTabEntries;
+    for (int i = threadIdx.x; i < kTabEntries; i += 256) {
+        const C d = diag_of<C>(i & 127, i >> 7, primary, c);   // x box faces from bits 7, 8
+        t[i] = d;
+        t[kTabEntries + i] = C(1) / d;
+    }
+}
+}  // namespace
+
+template <class T, class A>
+hipError_t build_tables(const Launch<T, A>& L, hipStream_t s) {
+    using C = typename CmpOf<T, A>::type;
+    if (L.nvol <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_tables<C>), dim3(L.nvol), dim3(256), 0, s, L.vol, L.coef, L.tab, L.nvol);
+    return hipGetLastError();
+}
+template hipError_t build_tables(const Launch<double, double>&, hipStream_t);
+template hipError_t build_tables(const Launch<float, float>&, hipStream_t);
+template hipError_t build_tables(const Launch<float, double>&, hipStream_t);
+
 // ---- pass launchers ----------------------------------------------------------------------
 #define FT_SHAPE_CASE(TXV, TYV, CALL) \
     if (L.tx == TXV && L.ty == TYV) { CALL(TXV, TYV); }
@@ -1050,7 +1179,7 @@ hipError_t Passes<T, A>::init_bicg(const Launch<T, A>& L, hipStream_t s) {
 #define FT_CALL(TXV, TYV)                                                                                        \
     hipLaunchKernelGGL((k_stencil<T, A, TXV, TYV, K_INIT_BICG>), dim3(L.ntiles), dim3(TXV * TYV), 0, s, L,       \
                        (const T*)nullptr, (T*)nullptr);                                                          \
-    hipLaunchKernelGGL((k_fin<T, A, F_INIT_BICG>), dim3(L.nvol), dim3(64), 0, s, L);                             \
+    hipLaunchKernelGGL((k_fin<T, A, F_INIT_BICG>), dim3(L.nvol), dim3(kFinThreads), 0, s, L);                    \
     return hipGetLastError();
 #define FT_TRY(TXV, TYV) FT_SHAPE_CASE(TXV, TYV, FT_CALL)
     FT_SHAPES(FT_TRY)
@@ -1063,7 +1192,7 @@ hipError_t Passes<T, A>::init_cg(const Launch<T, A>& L, hipStream_t s) {
 #define FT_CALL(TXV, TYV)                                                                                        \
     hipLaunchKernelGGL((k_stencil<T, A, TXV, TYV, K_INIT_CG>), dim3(L.ntiles), dim3(TXV * TYV), 0, s, L,         \
                        (const T*)nullptr, (T*)nullptr);                                                          \
-    hipLaunchKernelGGL((k_fin<T, A, F_INIT_CG>), dim3(L.nvol), dim3(64), 0, s, L);                               \
+    hipLaunchKernelGGL((k_fin<T, A, F_INIT_CG>), dim3(L.nvol), dim3(kFinThreads), 0, s, L);                    \
     return hipGetLastError();
     FT_SHAPES(FT_TRY)
 #undef FT_CALL
@@ -1075,7 +1204,7 @@ hipError_t Passes<T, A>::restart(const Launch<T, A>& L, hipStream_t s) {
 #define FT_CALL(TXV, TYV)                                                                                        \
     hipLaunchKernelGGL((k_stencil<T, A, TXV, TYV, K_RESTART>), dim3(L.ntiles), dim3(TXV * TYV), 0, s, L,         \
                        (const T*)nullptr, (T*)nullptr);                                                          \
-    hipLaunchKernelGGL((k_fin<T, A, F_RESTART>), dim3(L.nvol), dim3(64), 0, s, L);                               \
+    hipLaunchKernelGGL((k_fin<T, A, F_RESTART>), dim3(L.nvol), dim3(kFinThreads), 0, s, L);                    \
     return hipGetLastError();
     FT_SHAPES(FT_TRY)
 #undef FT_CALL
@@ -1134,6 +1263,7 @@ namespace {
 
 struct MixCtx {
     Coef<double> c;
+    const double* tab;
     int primary, sgn, np, first, fresh, reshadow, solver;
     const uint8_t* code;
     double* x64;
@@ -1145,23 +1275,31 @@ struct MixCtx {
 
 // FP64 true residual of the mixed form: u = x64 (x0 on the first step); r64 = b - A x64
 // replaces the FP32 residual; a fresh start also resets r0 = r and p = v = 0.
+struct MixRaw {
+    double a;   // x64
+    float e;    // r0
+    int code;
+};
+
 struct MixOps {
     static constexpr int NP = 4;   // |r64|^2, |b|^2, |float(r64)|^2, r0.float(r64)
+    using R = MixRaw;
     template <class X>
-    __device__ static Raw<double> fetch(const X& pc, long long idx, long long lin, bool centre) {
-        Raw<double> w;
-        if (!pc.first) w.a = pc.x64[idx];
-        if (centre && !pc.reshadow) w.e = (double)pc.r0[idx];
-        w.code = pc.code[lin];
+    __device__ static R fetch(const X& pc, long long idx, long long lin, bool centre) {
+        R w;
+        if (!pc.first) w.a = gl(pc.x64, idx);
+        if (centre && !pc.reshadow) w.e = gl(pc.r0, idx);
+        w.code = gl(pc.code, lin);
         return w;
     }
     template <class X>
-    __device__ static Pt<double> compute(const X& pc, const Raw<double>& w, int bx, int k) {
+    __device__ static Pt<double> compute(const X& pc, const R& w, int bx, int k) {
         Pt<double> p;
         p.code = w.code;
         p.u = pc.first ? x0_of<double>(k, pc.np, pc.sgn) : w.a;
-        p.dg = diag_of<double>(w.code, bx, pc.primary, pc.c);
-        p.e = w.e;
+        p.dg = diag_at<double>(pc.tab, w.code, bx, pc.primary, pc.c);
+        p.iv = inv_at<double>(pc.tab, w.code, bx, pc.primary, pc.c);
+        p.e = (double)w.e;
         return p;
     }
     template <class X, class Acc>
@@ -1180,11 +1318,13 @@ struct MixOps {
         acc[1].add(b, b);
         acc[2].add(r32, r32);
         if (pc.solver == 1)
-            acc[3].add((double)r32, (1.0 / pt.dg) * (double)r32);   // CG: r.(M^-1 r)
+            acc[3].add((double)r32, pt.iv * (double)r32);   // CG: r.(M^-1 r)
         else
             acc[3].add(pc.reshadow ? (double)r32 : pt.e, (double)r32);   // BiCGSTAB: r0.r
     }
 };
+
+__device__ void mixres_update(VolState<double>* s, const double* tot);
 
 template <int TX, int TY>
 __global__ __launch_bounds__(TX* TY) void k_mixres(Launch<float, double> L, const Coef<double>* coef64) {
@@ -1195,8 +1335,10 @@ __global__ __launch_bounds__(TX* TY) void k_mixres(Launch<float, double> L, cons
     const VolDesc& vd = L.vol[v];
     const VolState<double>* st = L.st + v;
     if (st->status != ST_UPDATE && st->status != ST_FRESH) return;
+    __shared__ double tab[2 * kTabEntries];
     MixCtx pc;
     pc.c = coef64[v];
+    pc.tab = tab;
     pc.primary = vd.primary;
     pc.sgn = vd.sgn;
     pc.np = dim_of(vd, vd.primary);
@@ -1211,6 +1353,7 @@ __global__ __launch_bounds__(TX* TY) void k_mixres(Launch<float, double> L, cons
     pc.r0 = (float*)L.vs.r0;
     pc.p_cur = (float*)(odd ? L.vs.p[1] : L.vs.p[0]);
     pc.v_cur = (float*)(odd ? L.vs.v[1] : L.vs.v[0]);
+    load_tables<double, TX * TY>(L.tab + (long long)v * 2 * kTabEntries, tab);
     AccDD acc[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) acc[k].init();
@@ -1218,29 +1361,23 @@ __global__ __launch_bounds__(TX* TY) void k_mixres(Launch<float, double> L, cons
     block_reduce_store<AccDD, 4, TX * TY>(acc, L.part + (long long)tile * kPart);
 }
 
-// Finalize of a reliable update / fresh start: the FP64 test decides whether the volume is
-// finished; otherwise BiCGSTAB continues on the replaced residual (a fresh start takes the
-// Eigen initial state with r0 = r).
-__global__ __launch_bounds__(64) void k_fin_mixres(Launch<float, double> L) {
+// One block per volume: reduce the partials of the reliable-update pass and decide.
+__global__ __launch_bounds__(kFinThreads) void k_fin_mixres(Launch<float, double> L) {
     const int v = blockIdx.x;
     if (v >= L.nvol) return;
     VolState<double>* s = L.st + v;
-    const int status = s->status;
-    if (status != ST_UPDATE && status != ST_FRESH) return;
+    if (s->status != ST_UPDATE && s->status != ST_FRESH) return;
     const VolDesc& vd = L.vol[v];
-    const int lane = threadIdx.x;
-    AccDD sums[4];
-    for (int k = 0; k < 4; ++k) sums[k].init();
-    for (int t = lane; t < vd.ntiles; t += 64) {
-        const double* p = L.part + (long long)(vd.tile_base + t) * kPart;
-        for (int k = 0; k < 4; ++k) sums[k].add2(p[2 * k], p[2 * k + 1]);
-    }
     double tot[4];
-    for (int k = 0; k < 4; ++k) {
-        wave_combine(sums[k]);
-        tot[k] = sums[k].hi + sums[k].lo;
-    }
-    if (lane != 0) return;
+    reduce_volume<4, kFinThreads>(L.part, vd.tile_base, vd.ntiles, tot);
+    if (threadIdx.x == 0) mixres_update(s, tot);
+}
+
+// Finalize of a reliable update / fresh start: the FP64 test decides whether the volume is
+// finished; otherwise BiCGSTAB continues on the replaced residual (a fresh start takes the
+// Eigen initial state with r0 = r).
+__device__ void mixres_update(VolState<double>* s, const double* tot) {
+    const int status = s->status;
     if (s->refinements == 0) {
         s->b64sq = tot[1];
         s->tol2 = s->otol * s->otol * s->b64sq;
@@ -1300,7 +1437,7 @@ __global__ __launch_bounds__(64) void k_fin_mixres(Launch<float, double> L) {
 hipError_t mixed_residual(const Launch<float, double>& L, const Coef<double>* coef64, hipStream_t s) {
 #define FT_CALL(TXV, TYV)                                                                                    \
     hipLaunchKernelGGL((k_mixres<TXV, TYV>), dim3(L.ntiles), dim3(TXV * TYV), 0, s, L, coef64);             \
-    hipLaunchKernelGGL(k_fin_mixres, dim3(L.nvol), dim3(64), 0, s, L);                                       \
+    hipLaunchKernelGGL(k_fin_mixres, dim3(L.nvol), dim3(kFinThreads), 0, s, L);                               \
     return hipGetLastError();
 #define FT_TRY(TXV, TYV) FT_SHAPE_CASE(TXV, TYV, FT_CALL)
     FT_SHAPES(FT_TRY)
