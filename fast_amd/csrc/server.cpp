@@ -478,7 +478,7 @@ int fast_server_submit(fast_server* s, const uint8_t* mask, int nx, int ny, int 
     if (!s || !mask || !params || !ticket || nx <= 0 || ny <= 0 || nz <= 0) return FAST_E_BADARG;
     if (params->dir < 0 || params->dir > 5 || !(params->tolerance > 0) || params->max_iter < 0) return FAST_E_BADARG;
     const size_t n = (size_t)nx * ny * nz;
-    if (n >= ((size_t)1 << 31)) return FAST_E_BADARG;
+    if (n > kMaxSolveVoxels) return FAST_E_BADARG;
     auto* r = new Req();
     r->mask.assign(mask, mask + n);
     r->nx = nx;

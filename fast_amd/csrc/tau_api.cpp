@@ -464,6 +464,11 @@ static int prepare(Batch<T, A>& B) {
     for (int i = 0; i < nv; ++i) {
         const Job& j = *B.jobs[i];
         const fast_volume* v = j.vol;
+        if ((size_t)v->nx * v->ny * v->nz > kMaxSolveVoxels) {
+            set_error(ctx, "volume of %d x %d x %d voxels exceeds the solver's limit of %zu voxels", v->nx, v->ny, v->nz,
+                      kMaxSolveVoxels);
+            return FAST_E_BADARG;
+        }
         VolDesc& d = B.hvol[i];
         d.nx = v->nx;
         d.ny = v->ny;

@@ -16,6 +16,10 @@ namespace fth {
 // between, so its trajectory does not depend on this interval.
 constexpr int kMixedUpdateEvery = 4;
 
+// The stencil kernels address each vector of a volume with 32-bit byte offsets, so an FP64
+// vector (8 bytes per voxel) must stay below 4 GiB.
+constexpr size_t kMaxSolveVoxels = ((size_t)1 << 29) - 1;
+
 struct Shape {
     int tx, ty, tz;
 };

@@ -162,25 +162,39 @@ __device__ __forceinline__ C inv_at(const C* tab, int code, int bx, int primary,
     return tab[kTabEntries + (code | ((bx & 3) << 7))];
 }
 
+// Coefficients of the + and - face of one axis for (own, nbr+, nbr-) = index bits (2, 0, 1):
+// ci[own + nbr], the interior-face value (Diffusion.cuh:85-105).  Eight entries per volume,
+// kept in LDS so a row reads its three pairs instead of selecting six values.
+template <class T>
+struct FacePair {
+    T p, m;
+};
+
+template <class T>
+__device__ __forceinline__ FacePair<T> face_pair(const Coef<T>& c, int e) {
+    const int own = (e >> 2) & 1;
+    return FacePair<T>{face_val(c, own, e & 1), face_val(c, own, (e >> 1) & 1)};
+}
+
 // Row product in CSR column order Z-,Y-,X-,diag,X+,Y+,Z+.  A box face has no off-diagonal
 // coefficient in the reference; here its term is c * (+0): the plane march supplies +0 for
 // every neighbour outside the box, c is finite and positive, and the accumulator (started
 // at +0) is never -0, so the term leaves the sum unchanged bit for bit.
 template <class T>
-__device__ __forceinline__ T apply_row(int code, T dg, const Coef<T>& c, T uzm, T uym, T uxm, T uc, T uxp, T uyp,
-                                       T uzp) {
-    // face coefficient ci[own + nbr]: choose the pair by the own flag once, then per face
-    const bool own = ((code >> 6) & 1) != 0;
-    const T lo = pick2(own, c.ci0, c.ci1);   // neighbour pore
-    const T hi = pick2(own, c.ci1, c.ci2);   // neighbour solid
+__device__ __forceinline__ T apply_row(int code, T dg, const FacePair<T>* ftab, T uzm, T uym, T uxm, T uc, T uxp,
+                                       T uyp, T uzp) {
+    const int o4 = (code >> 4) & 4;   // own solid flag (bit 6) as bit 2 of the pair index
+    const FacePair<T> px = ftab[(code & 3) | o4];
+    const FacePair<T> py = ftab[((code >> 2) & 3) | o4];
+    const FacePair<T> pz = ftab[((code >> 4) & 3) | o4];
     T acc = T(0);
-    acc += pick2(((code >> 5) & 1) != 0, lo, hi) * uzm;
-    acc += pick2(((code >> 3) & 1) != 0, lo, hi) * uym;
-    acc += pick2(((code >> 1) & 1) != 0, lo, hi) * uxm;
+    acc += pz.m * uzm;
+    acc += py.m * uym;
+    acc += px.m * uxm;
     acc += dg * uc;
-    acc += pick2((code & 1) != 0, lo, hi) * uxp;
-    acc += pick2(((code >> 2) & 1) != 0, lo, hi) * uyp;
-    acc += pick2(((code >> 4) & 1) != 0, lo, hi) * uzp;
+    acc += px.p * uxp;
+    acc += py.p * uyp;
+    acc += pz.p * uzp;
     return acc;
 }
 
@@ -261,11 +275,22 @@ __device__ __forceinline__ void block_reduce_store(Acc (&acc)[NP], double* out) 
 // (and the centre values the epilogue needs).  epi(): outputs and partial sums at the centre.
 // fetch and compute are split so the plane march can issue loads two planes ahead.
 // ---------------------------------------------------------------------------------------
-// Load through a global-address-space pointer (a flat load would also count against the LDS
-// counter and be waited for at every barrier of the plane march).
+// Element i of one volume's vector: a uniform base pointer plus a 32-bit byte offset in the
+// global address space, so loads and stores take the SGPR-base form with no 64-bit address
+// arithmetic per element (a flat access would also be counted against the LDS counter and
+// waited for at every barrier).  Every vector of a volume stays below 4 GiB (checked on the
+// host).
 template <class T>
-__device__ __forceinline__ T gl(const T* p, long long i) {
-    return ((const __attribute__((address_space(1))) T*)p)[i];
+__device__ __forceinline__ T ld(const T* base, unsigned i) {
+    using GT = const __attribute__((address_space(1))) T;
+    using GC = const __attribute__((address_space(1))) char;
+    return *(GT*)((GC*)base + i * (unsigned)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ void st(T* base, unsigned i, T v) {
+    using GT = __attribute__((address_space(1))) T;
+    using GC = __attribute__((address_space(1))) char;
+    *(GT*)((GC*)base + i * (unsigned)sizeof(T)) = v;
 }
 
 template <class T>
@@ -288,7 +313,8 @@ template <class T, class A>
 struct PassCtx {
     using C = typename CmpOf<T, A>::type;
     Coef<C> c;
-    const C* tab;   // diagonal tables of the volume (LDS)
+    const C* tab;                // diagonal tables of the volume (LDS)
+    const FacePair<C>* ftab;     // face coefficient pairs (LDS)
     int primary, sgn, np, first;
     C alpha, beta, omega;
     const uint8_t* code;
@@ -312,9 +338,9 @@ struct PassOps<T, A, K_INIT_BICG> {
     using R = Raw<T>;   // raw loads in the storage type; widened in compute()
     static constexpr int NP = 2;   // |r|^2, |b|^2
     template <class X>
-    __device__ static R fetch(const X& pc, long long idx, long long lin, bool centre) {
+    __device__ static R fetch(const X& pc, unsigned i, bool centre) {
         R w;
-        w.code = gl(pc.code, lin);
+        w.code = ld(pc.code, i);
         return w;
     }
     template <class X>
@@ -326,14 +352,14 @@ struct PassOps<T, A, K_INIT_BICG> {
         return p;
     }
     template <class X, class Acc>
-    __device__ static void epi(const X& pc, long long idx, int k, C Au, const Pt<C>& pt, Acc (&acc)[NP]) {
+    __device__ static void epi(const X& pc, unsigned i, int k, C Au, const Pt<C>& pt, Acc (&acc)[NP]) {
         const C b = rhs_of<C>(pt.code, k, pc.np, pc.c);
         const C r = b - Au;
-        pc.x[idx] = T(pt.u);
-        pc.r[idx] = T(r);
-        pc.r0[idx] = T(r);
-        pc.p_old[idx] = T(0);
-        pc.v_old[idx] = T(0);
+        st(pc.x, i, T(pt.u));
+        st(pc.r, i, T(r));
+        st(pc.r0, i, T(r));
+        st(pc.p_old, i, T(0));
+        st(pc.v_old, i, T(0));
         acc[0].add(r, r);
         acc[1].add(b, b);
     }
@@ -346,9 +372,9 @@ struct PassOps<T, A, K_INIT_CG> {
     using R = Raw<T>;   // raw loads in the storage type; widened in compute()
     static constexpr int NP = 3;   // |r|^2, |b|^2, r.(M^-1 r)
     template <class X>
-    __device__ static R fetch(const X& pc, long long idx, long long lin, bool centre) {
+    __device__ static R fetch(const X& pc, unsigned i, bool centre) {
         R w;
-        w.code = gl(pc.code, lin);
+        w.code = ld(pc.code, i);
         return w;
     }
     template <class X>
@@ -361,12 +387,12 @@ struct PassOps<T, A, K_INIT_CG> {
         return p;
     }
     template <class X, class Acc>
-    __device__ static void epi(const X& pc, long long idx, int k, C Au, const Pt<C>& pt, Acc (&acc)[NP]) {
+    __device__ static void epi(const X& pc, unsigned i, int k, C Au, const Pt<C>& pt, Acc (&acc)[NP]) {
         const C b = rhs_of<C>(pt.code, k, pc.np, pc.c);
         const C r = b - Au;
         const C inv = pt.iv;
-        pc.x[idx] = T(pt.u);
-        pc.r[idx] = T(r);
+        st(pc.x, i, T(pt.u));
+        st(pc.r, i, T(r));
         acc[0].add(r, r);
         acc[1].add(b, b);
         acc[2].add(r, inv * r);
@@ -380,10 +406,10 @@ struct PassOps<T, A, K_RESTART> {
     using R = Raw<T>;   // raw loads in the storage type; widened in compute()
     static constexpr int NP = 1;
     template <class X>
-    __device__ static R fetch(const X& pc, long long idx, long long lin, bool centre) {
+    __device__ static R fetch(const X& pc, unsigned i, bool centre) {
         R w;
-        w.a = gl(pc.x, idx);
-        w.code = gl(pc.code, lin);
+        w.a = ld(pc.x, i);
+        w.code = ld(pc.code, i);
         return w;
     }
     template <class X>
@@ -395,11 +421,11 @@ struct PassOps<T, A, K_RESTART> {
         return p;
     }
     template <class X, class Acc>
-    __device__ static void epi(const X& pc, long long idx, int k, C Au, const Pt<C>& pt, Acc (&acc)[NP]) {
+    __device__ static void epi(const X& pc, unsigned i, int k, C Au, const Pt<C>& pt, Acc (&acc)[NP]) {
         const C b = rhs_of<C>(pt.code, k, pc.np, pc.c);
         const C r = b - Au;
-        pc.r[idx] = T(r);
-        pc.r0[idx] = T(r);
+        st(pc.r, i, T(r));
+        st(pc.r0, i, T(r));
         acc[0].add(r, r);
     }
 };
@@ -411,10 +437,10 @@ struct PassOps<T, A, K_APPLY> {
     using R = Raw<T>;   // raw loads in the storage type; widened in compute()
     static constexpr int NP = 0;
     template <class X>
-    __device__ static R fetch(const X& pc, long long idx, long long lin, bool centre) {
+    __device__ static R fetch(const X& pc, unsigned i, bool centre) {
         R w;
-        w.a = gl(pc.x, idx);
-        w.code = gl(pc.code, lin);
+        w.a = ld(pc.x, i);
+        w.code = ld(pc.code, i);
         return w;
     }
     template <class X>
@@ -426,8 +452,8 @@ struct PassOps<T, A, K_APPLY> {
         return p;
     }
     template <class X, class Acc>
-    __device__ static void epi(const X& pc, long long idx, int k, C Au, const Pt<C>& pt, Acc (&)[1]) {
-        pc.s[idx] = T(Au);
+    __device__ static void epi(const X& pc, unsigned i, int k, C Au, const Pt<C>& pt, Acc (&)[1]) {
+        st(pc.s, i, T(Au));
     }
 };
 
@@ -438,13 +464,13 @@ struct PassOps<T, A, K_BICG1> {
     using R = Raw<T>;   // raw loads in the storage type; widened in compute()
     static constexpr int NP = 1;
     template <class X>
-    __device__ static R fetch(const X& pc, long long idx, long long lin, bool centre) {
+    __device__ static R fetch(const X& pc, unsigned i, bool centre) {
         R w;
-        w.a = gl(pc.r, idx);
-        w.b = gl(pc.p_old, idx);
-        w.c = gl(pc.v_old, idx);
-        if (centre) w.e = gl(pc.r0, idx);
-        w.code = gl(pc.code, lin);
+        w.a = ld(pc.r, i);
+        w.b = ld(pc.p_old, i);
+        w.c = ld(pc.v_old, i);
+        if (centre) w.e = ld(pc.r0, i);
+        w.code = ld(pc.code, i);
         return w;
     }
     template <class X>
@@ -461,9 +487,9 @@ struct PassOps<T, A, K_BICG1> {
         return q;
     }
     template <class X, class Acc>
-    __device__ static void epi(const X& pc, long long idx, int k, C Au, const Pt<C>& pt, Acc (&acc)[NP]) {
-        pc.p_new[idx] = T(pt.aux);
-        pc.v_new[idx] = T(Au);
+    __device__ static void epi(const X& pc, unsigned i, int k, C Au, const Pt<C>& pt, Acc (&acc)[NP]) {
+        st(pc.p_new, i, T(pt.aux));
+        st(pc.v_new, i, T(Au));
         acc[0].add(pt.e, Au);
     }
 };
@@ -477,11 +503,11 @@ struct PassOps<T, A, K_BICG2> {
     using R = Raw<T>;   // raw loads in the storage type; widened in compute()
     static constexpr int NP = 2;
     template <class X>
-    __device__ static R fetch(const X& pc, long long idx, long long lin, bool centre) {
+    __device__ static R fetch(const X& pc, unsigned i, bool centre) {
         R w;
-        w.a = gl(pc.r, idx);
-        w.b = gl(pc.v_new, idx);
-        w.code = gl(pc.code, lin);
+        w.a = ld(pc.r, i);
+        w.b = ld(pc.v_new, i);
+        w.code = ld(pc.code, i);
         return w;
     }
     template <class X>
@@ -496,8 +522,8 @@ struct PassOps<T, A, K_BICG2> {
         return q;
     }
     template <class X, class Acc>
-    __device__ static void epi(const X& pc, long long idx, int k, C Au, const Pt<C>& pt, Acc (&acc)[NP]) {
-        pc.s[idx] = T(pt.aux);
+    __device__ static void epi(const X& pc, unsigned i, int k, C Au, const Pt<C>& pt, Acc (&acc)[NP]) {
+        st(pc.s, i, T(pt.aux));
         acc[0].add(Au, pt.aux);
         acc[1].add(Au, Au);
     }
@@ -512,15 +538,15 @@ struct PassOps<T, A, K_BICG3> {
     using R = Raw<T>;   // raw loads in the storage type; widened in compute()
     static constexpr int NP = 2;
     template <class X>
-    __device__ static R fetch(const X& pc, long long idx, long long lin, bool centre) {
+    __device__ static R fetch(const X& pc, unsigned i, bool centre) {
         R w;
-        w.a = gl(pc.s, idx);
+        w.a = ld(pc.s, i);
         if (centre) {
-            w.c = gl(pc.p_new, idx);
-            w.e = gl(pc.r0, idx);
-            w.f = gl(pc.x, idx);
+            w.c = ld(pc.p_new, i);
+            w.e = ld(pc.r0, i);
+            w.f = ld(pc.x, i);
         }
-        w.code = gl(pc.code, lin);
+        w.code = ld(pc.code, i);
         return w;
     }
     template <class X>
@@ -537,11 +563,11 @@ struct PassOps<T, A, K_BICG3> {
         return q;
     }
     template <class X, class Acc>
-    __device__ static void epi(const X& pc, long long idx, int k, C Au, const Pt<C>& pt, Acc (&acc)[NP]) {
+    __device__ static void epi(const X& pc, unsigned i, int k, C Au, const Pt<C>& pt, Acc (&acc)[NP]) {
         const C yv = pt.iv * pt.g;
-        pc.x[idx] = T(pt.f + (pc.alpha * yv + pc.omega * pt.u));
+        st(pc.x, i, T(pt.f + (pc.alpha * yv + pc.omega * pt.u)));
         const T rs = T(pt.aux - pc.omega * Au);   // the stored residual is what the next iteration sees
-        pc.r[idx] = rs;
+        st(pc.r, i, rs);
         acc[0].add(C(rs), C(rs));
         acc[1].add(pt.e, C(rs));
     }
@@ -556,11 +582,11 @@ struct PassOps<T, A, K_CG1> {
     using R = Raw<T>;   // raw loads in the storage type; widened in compute()
     static constexpr int NP = 1;
     template <class X>
-    __device__ static R fetch(const X& pc, long long idx, long long lin, bool centre) {
+    __device__ static R fetch(const X& pc, unsigned i, bool centre) {
         R w;
-        w.a = gl(pc.r, idx);
-        w.b = gl(pc.p_old, idx);   // unused on the first iteration (an unconditional load stays in flight)
-        w.code = gl(pc.code, lin);
+        w.a = ld(pc.r, i);
+        w.b = ld(pc.p_old, i);   // unused on the first iteration (an unconditional load stays in flight)
+        w.code = ld(pc.code, i);
         return w;
     }
     template <class X>
@@ -575,8 +601,8 @@ struct PassOps<T, A, K_CG1> {
         return q;
     }
     template <class X, class Acc>
-    __device__ static void epi(const X& pc, long long idx, int k, C Au, const Pt<C>& pt, Acc (&acc)[NP]) {
-        pc.p_new[idx] = T(pt.u);
+    __device__ static void epi(const X& pc, unsigned i, int k, C Au, const Pt<C>& pt, Acc (&acc)[NP]) {
+        st(pc.p_new, i, T(pt.u));
         acc[0].add(pt.u, Au);
     }
 };
@@ -589,14 +615,14 @@ struct PassOps<T, A, K_CG2> {
     using R = Raw<T>;   // raw loads in the storage type; widened in compute()
     static constexpr int NP = 2;
     template <class X>
-    __device__ static R fetch(const X& pc, long long idx, long long lin, bool centre) {
+    __device__ static R fetch(const X& pc, unsigned i, bool centre) {
         R w;
-        w.a = gl(pc.p_new, idx);
+        w.a = ld(pc.p_new, i);
         if (centre) {
-            w.b = gl(pc.x, idx);
-            w.c = gl(pc.r, idx);
+            w.b = ld(pc.x, i);
+            w.c = ld(pc.r, i);
         }
-        w.code = gl(pc.code, lin);
+        w.code = ld(pc.code, i);
         return w;
     }
     template <class X>
@@ -611,10 +637,10 @@ struct PassOps<T, A, K_CG2> {
         return q;
     }
     template <class X, class Acc>
-    __device__ static void epi(const X& pc, long long idx, int k, C Au, const Pt<C>& pt, Acc (&acc)[NP]) {
-        pc.x[idx] = T(pt.aux + pc.alpha * pt.u);
+    __device__ static void epi(const X& pc, unsigned i, int k, C Au, const Pt<C>& pt, Acc (&acc)[NP]) {
+        st(pc.x, i, T(pt.aux + pc.alpha * pt.u));
         const T rs = T(pt.e - pc.alpha * Au);   // the stored residual is what the next iteration sees
-        pc.r[idx] = rs;
+        st(pc.r, i, rs);
         acc[0].add(C(rs), C(rs));
         acc[1].add(C(rs), pt.iv * C(rs));
     }
@@ -670,10 +696,9 @@ __device__ __forceinline__ void march(const Ctx& pc, const VolDesc& vd, int lt, 
     // outside the volume (or past the last plane) loads is never used.
     const int xc = min(x, nx - 1), yc = min(y, ny - 1);
     const int hxc = min(max(hx, 0), nx - 1), hyc = min(max(hy, 0), ny - 1);
-    const long long plane = (long long)nx * ny;
-    const long long clin = (long long)xc + (long long)nx * yc;    // centre column
-    const long long hlin = (long long)hxc + (long long)nx * hyc;  // halo column
-    const long long base = vd.off;
+    const unsigned plane = (unsigned)nx * (unsigned)ny;
+    const unsigned clin = (unsigned)xc + (unsigned)nx * (unsigned)yc;    // centre column
+    const unsigned hlin = (unsigned)hxc + (unsigned)nx * (unsigned)hyc;  // halo column
     auto bxz = [&](int bxy, int pz) { return bxy | (pz == nz - 1 ? 16 : 0) | (pz == 0 ? 32 : 0); };
     const int cbxy = (x == nx - 1 ? 1 : 0) | (x == 0 ? 2 : 0) | (y == ny - 1 ? 4 : 0) | (y == 0 ? 8 : 0);
     const int hbxy = (hx == nx - 1 ? 1 : 0) | (hx == 0 ? 2 : 0) | (hy == ny - 1 ? 4 : 0) | (hy == 0 ? 8 : 0);
@@ -684,20 +709,15 @@ __device__ __forceinline__ void march(const Ctx& pc, const VolDesc& vd, int lt, 
     Pt<U> c_cur{};
     R rcA{}, rcB{}, rhA{}, rhB{};
     if (z0 > 0) {
-        const long long l = clin + plane * (z0 - 1);
-        u_prev = Ops::compute(pc, Ops::fetch(pc, base + l, l, false), bxz(cbxy, z0 - 1), kc(x, y, z0 - 1)).u;
+        u_prev = Ops::compute(pc, Ops::fetch(pc, clin + plane * (z0 - 1), false), bxz(cbxy, z0 - 1), kc(x, y, z0 - 1)).u;
     }
     {
-        const long long l0 = clin + plane * z0;
-        c_cur = Ops::compute(pc, Ops::fetch(pc, base + l0, l0, true), bxz(cbxy, z0), kc(x, y, z0));
-        const long long l1 = clin + plane * clampz(z0 + 1);
-        rcA = Ops::fetch(pc, base + l1, l1, true);
+        c_cur = Ops::compute(pc, Ops::fetch(pc, clin + plane * z0, true), bxz(cbxy, z0), kc(x, y, z0));
+        rcA = Ops::fetch(pc, clin + plane * clampz(z0 + 1), true);
     }
     if (hwave) {
-        const long long l0 = hlin + plane * z0;
-        h_cur = Ops::compute(pc, Ops::fetch(pc, base + l0, l0, false), bxz(hbxy, z0), kc(hx, hy, z0)).u;
-        const long long l1 = hlin + plane * clampz(z0 + 1);
-        rhA = Ops::fetch(pc, base + l1, l1, false);
+        h_cur = Ops::compute(pc, Ops::fetch(pc, hlin + plane * z0, false), bxz(hbxy, z0), kc(hx, hy, z0)).u;
+        rhA = Ops::fetch(pc, hlin + plane * clampz(z0 + 1), false);
     }
 
     // One plane: `rc_in` / `rh_in` hold the loads of plane z+1 (issued one plane ago), the
@@ -708,24 +728,18 @@ __device__ __forceinline__ void march(const Ctx& pc, const VolDesc& vd, int lt, 
         const Pt<U> c_next = Ops::compute(pc, rc_in, bxz(cbxy, z + 1), kc(x, y, z + 1));
         U h_next = U(0);
         if (hwave) h_next = Ops::compute(pc, rh_in, bxz(hbxy, z + 1), kc(hx, hy, z + 1)).u;
-        {
-            const long long l = clin + plane * clampz(z + 2);
-            rc_out = Ops::fetch(pc, base + l, l, true);
-        }
-        if (hwave) {
-            const long long l = hlin + plane * clampz(z + 2);
-            rh_out = Ops::fetch(pc, base + l, l, false);
-        }
+        rc_out = Ops::fetch(pc, clin + plane * clampz(z + 2), true);
+        if (hwave) rh_out = Ops::fetch(pc, hlin + plane * clampz(z + 2), false);
         U* buf = lds + (z & 1) * PL;
         buf[(ty + 1) * W + (tx + 1)] = inside ? c_cur.u : U(0);
         if (hl >= 0) buf[hl] = hvalid ? h_cur : U(0);
         __syncthreads();
         if (inside) {
             const U uzp = has_next ? c_next.u : U(0);
-            const U Au = apply_row<U>(c_cur.code, c_cur.dg, pc.c, u_prev, buf[ty * W + (tx + 1)],
+            const U Au = apply_row<U>(c_cur.code, c_cur.dg, pc.ftab, u_prev, buf[ty * W + (tx + 1)],
                                       buf[(ty + 1) * W + tx], c_cur.u, buf[(ty + 1) * W + (tx + 2)],
                                       buf[(ty + 2) * W + (tx + 1)], uzp);
-            Ops::epi(pc, base + clin + plane * z, kc(x, y, z), Au, c_cur, acc);
+            Ops::epi(pc, clin + plane * z, kc(x, y, z), Au, c_cur, acc);
         }
         u_prev = c_cur.u;
         c_cur = c_next;
@@ -772,10 +786,11 @@ __device__ __forceinline__ void reduce_volume(const double* part, int base, int 
     }
 }
 
-// Copy the volume's diagonal tables into LDS.
+// Copy the volume's diagonal tables into LDS and build its face-pair table there.
 template <class C, int NT>
-__device__ __forceinline__ void load_tables(const C* src, C* dst) {
+__device__ __forceinline__ void load_tables(const C* src, C* dst, FacePair<C>* fdst, const Coef<C>& c) {
     for (int i = threadIdx.x; i < 2 * kTabEntries; i += NT) dst[i] = src[i];
+    if (threadIdx.x < 8) fdst[threadIdx.x] = face_pair(c, threadIdx.x);
     __syncthreads();
 }
 
@@ -787,6 +802,7 @@ __global__ __launch_bounds__(TX* TY) void k_stencil(Launch<T, A> L, const T* xin
     using C = typename CmpOf<T, A>::type;
     __shared__ C lds[2 * (TY + 2) * (TX + 2)];
     __shared__ C tab[2 * kTabEntries];
+    __shared__ FacePair<C> ftab[8];
 
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
     if (tile >= L.ntiles) return;
@@ -799,6 +815,7 @@ __global__ __launch_bounds__(TX* TY) void k_stencil(Launch<T, A> L, const T* xin
     PassCtx<T, A> pc;
     pc.c = L.coef[v];
     pc.tab = tab;
+    pc.ftab = ftab;
     pc.primary = vd.primary;
     pc.sgn = vd.sgn;
     pc.np = dim_of(vd, vd.primary);
@@ -808,21 +825,23 @@ __global__ __launch_bounds__(TX* TY) void k_stencil(Launch<T, A> L, const T* xin
     pc.omega = C(st->omega);
     pc.code = vd.code;
     {
+        // per-volume base pointers: element offsets below are within the volume
         const bool odd = (st->phase & 1) != 0;
-        pc.x = (T*)L.vs.x;
-        pc.r = (T*)L.vs.r;
-        pc.r0 = (T*)L.vs.r0;
-        pc.p_old = (T*)(odd ? L.vs.p[1] : L.vs.p[0]);
-        pc.p_new = (T*)(odd ? L.vs.p[0] : L.vs.p[1]);
-        pc.v_old = (T*)(odd ? L.vs.v[1] : L.vs.v[0]);
-        pc.v_new = (T*)(odd ? L.vs.v[0] : L.vs.v[1]);
-        pc.s = (T*)L.vs.s;
+        const long long o = vd.off;
+        pc.x = (T*)L.vs.x + o;
+        pc.r = (T*)L.vs.r + o;
+        pc.r0 = (T*)L.vs.r0 + o;
+        pc.p_old = (T*)(odd ? L.vs.p[1] : L.vs.p[0]) + o;
+        pc.p_new = (T*)(odd ? L.vs.p[0] : L.vs.p[1]) + o;
+        pc.v_old = (T*)(odd ? L.vs.v[1] : L.vs.v[0]) + o;
+        pc.v_new = (T*)(odd ? L.vs.v[0] : L.vs.v[1]) + o;
+        pc.s = (T*)L.vs.s + o;
         if (KIND == K_APPLY) {
-            pc.x = (T*)xin;
-            pc.s = yout;
+            pc.x = (T*)xin + o;
+            pc.s = yout + o;
         }
     }
-    load_tables<C, TX * TY>(L.tab + (long long)v * 2 * kTabEntries, tab);
+    load_tables<C, TX * TY>(L.tab + (long long)v * 2 * kTabEntries, tab, ftab, pc.c);
     using Acc = typename AccOf<T, A>::type;
     Acc acc[NPA];
 #pragma unroll
@@ -1264,6 +1283,7 @@ namespace {
 struct MixCtx {
     Coef<double> c;
     const double* tab;
+    const FacePair<double>* ftab;
     int primary, sgn, np, first, fresh, reshadow, solver;
     const uint8_t* code;
     double* x64;
@@ -1285,11 +1305,11 @@ struct MixOps {
     static constexpr int NP = 4;   // |r64|^2, |b|^2, |float(r64)|^2, r0.float(r64)
     using R = MixRaw;
     template <class X>
-    __device__ static R fetch(const X& pc, long long idx, long long lin, bool centre) {
+    __device__ static R fetch(const X& pc, unsigned i, bool centre) {
         R w;
-        if (!pc.first) w.a = gl(pc.x64, idx);
-        if (centre && !pc.reshadow) w.e = gl(pc.r0, idx);
-        w.code = gl(pc.code, lin);
+        if (!pc.first) w.a = ld(pc.x64, i);
+        if (centre && !pc.reshadow) w.e = ld(pc.r0, i);
+        w.code = ld(pc.code, i);
         return w;
     }
     template <class X>
@@ -1303,16 +1323,16 @@ struct MixOps {
         return p;
     }
     template <class X, class Acc>
-    __device__ static void epi(const X& pc, long long idx, int k, double Au, const Pt<double>& pt, Acc (&acc)[NP]) {
+    __device__ static void epi(const X& pc, unsigned i, int k, double Au, const Pt<double>& pt, Acc (&acc)[NP]) {
         const double b = rhs_of<double>(pt.code, k, pc.np, pc.c);
         const double r64 = b - Au;
         const float r32 = (float)r64;
-        if (pc.first) pc.x64[idx] = pt.u;
-        pc.r[idx] = r32;
-        if (pc.reshadow) pc.r0[idx] = r32;
+        if (pc.first) st(pc.x64, i, pt.u);
+        st(pc.r, i, r32);
+        if (pc.reshadow) st(pc.r0, i, r32);
         if (pc.fresh) {
-            pc.p_cur[idx] = 0.0f;
-            pc.v_cur[idx] = 0.0f;
+            st(pc.p_cur, i, 0.0f);
+            st(pc.v_cur, i, 0.0f);
         }
         acc[0].add(r64, r64);
         acc[1].add(b, b);
@@ -1336,9 +1356,11 @@ __global__ __launch_bounds__(TX* TY) void k_mixres(Launch<float, double> L, cons
     const VolState<double>* st = L.st + v;
     if (st->status != ST_UPDATE && st->status != ST_FRESH) return;
     __shared__ double tab[2 * kTabEntries];
+    __shared__ FacePair<double> ftab[8];
     MixCtx pc;
     pc.c = coef64[v];
     pc.tab = tab;
+    pc.ftab = ftab;
     pc.primary = vd.primary;
     pc.sgn = vd.sgn;
     pc.np = dim_of(vd, vd.primary);
@@ -1348,12 +1370,13 @@ __global__ __launch_bounds__(TX* TY) void k_mixres(Launch<float, double> L, cons
     pc.reshadow = (pc.fresh || st->reshadow) && pc.solver == 0;
     pc.code = vd.code;
     const bool odd = (st->phase & 1) != 0;
-    pc.x64 = L.vs.x64;
-    pc.r = (float*)L.vs.r;
-    pc.r0 = (float*)L.vs.r0;
-    pc.p_cur = (float*)(odd ? L.vs.p[1] : L.vs.p[0]);
-    pc.v_cur = (float*)(odd ? L.vs.v[1] : L.vs.v[0]);
-    load_tables<double, TX * TY>(L.tab + (long long)v * 2 * kTabEntries, tab);
+    const long long o = vd.off;
+    pc.x64 = L.vs.x64 + o;
+    pc.r = (float*)L.vs.r + o;
+    pc.r0 = (float*)L.vs.r0 + o;
+    pc.p_cur = (float*)(odd ? L.vs.p[1] : L.vs.p[0]) + o;
+    pc.v_cur = (float*)(odd ? L.vs.v[1] : L.vs.v[0]) + o;
+    load_tables<double, TX * TY>(L.tab + (long long)v * 2 * kTabEntries, tab, ftab, pc.c);
     AccDD acc[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) acc[k].init();

@@ -127,6 +127,7 @@ int fast_volume_count_zero(fast_volume* vol, uint64_t* count);
 int fast_porosity(fast_ctx* ctx, const uint8_t* mask, int nx, int ny, int nz, double* porosity);
 
 /* getTortuosity<T> (VolumeMeasures.cpp:42-285): assemble (matrix-free), solve, flux, tau.
+ * Volumes of up to 2^29 - 1 voxels (e.g. 812^3) are solved; larger ones return FAST_E_BADARG.
  * conc_out (nullable, N doubles) receives the concentration field (the reference's
  * concetrationOutput).  Returns FAST_OK, FAST_E_NOCONV (result->tau = 0) or an error. */
 int fast_tau_solve(fast_ctx* ctx, const uint8_t* mask, int nx, int ny, int nz,
