@@ -37,6 +37,10 @@ namespace {
 
 constexpr int kPart = 8;         // partial-sum doubles per tile: up to 4 sums as (hi, lo) pairs
 constexpr int kFinThreads = 256; // threads of a per-volume finalize block
+#ifndef FT_PREFETCH
+#define FT_PREFETCH 2
+#endif
+constexpr int kPrefetch = FT_PREFETCH;   // planes the stencil loads run ahead of their use
 
 enum PassKind : int {
     K_INIT_BICG = 0,
@@ -151,15 +155,18 @@ __device__ __forceinline__ T x0_of(int k, int np, int sgn) {
 // tiles the branch is uniform over a wave.  The tables were filled by diag_of itself, so both
 // ways give the same bits.
 template <class C>
-__device__ __forceinline__ C diag_at(const C* tab, int code, int bx, int primary, const Coef<C>& c) {
-    if (bx & 0x3C) return diag_of<C>(code, bx, primary, c);
-    return tab[code | ((bx & 3) << 7)];
-}
+struct DiagInv {
+    C d, iv;   // diagonal and its inverse
+};
 
 template <class C>
-__device__ __forceinline__ C inv_at(const C* tab, int code, int bx, int primary, const Coef<C>& c) {
-    if (bx & 0x3C) return C(1) / diag_of<C>(code, bx, primary, c);
-    return tab[kTabEntries + (code | ((bx & 3) << 7))];
+__device__ __forceinline__ DiagInv<C> diag_at(const C* tab, int code, int bx, int primary, const Coef<C>& c) {
+    if (bx & 0x3C) {
+        const C d = diag_of<C>(code, bx, primary, c);
+        return DiagInv<C>{d, C(1) / d};
+    }
+    const int i = code | ((bx & 3) << 7);
+    return DiagInv<C>{tab[i], tab[kTabEntries + i]};
 }
 
 // Coefficients of the + and - face of one axis for (own, nbr+, nbr-) = index bits (2, 0, 1):
@@ -348,7 +355,7 @@ struct PassOps<T, A, K_INIT_BICG> {
         Pt<C> p;
         p.code = w.code;
         p.u = x0_of<C>(k, pc.np, pc.sgn);
-        p.dg = diag_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
+        p.dg = diag_at<C>(pc.tab, w.code, bx, pc.primary, pc.c).d;
         return p;
     }
     template <class X, class Acc>
@@ -382,8 +389,11 @@ struct PassOps<T, A, K_INIT_CG> {
         Pt<C> p;
         p.code = w.code;
         p.u = x0_of<C>(k, pc.np, pc.sgn);
-        p.dg = diag_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
-        p.iv = inv_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
+        {
+            const DiagInv<C> di = diag_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
+            p.dg = di.d;
+            p.iv = di.iv;
+        }
         return p;
     }
     template <class X, class Acc>
@@ -417,7 +427,7 @@ struct PassOps<T, A, K_RESTART> {
         Pt<C> p;
         p.code = w.code;
         p.u = C(w.a);
-        p.dg = diag_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
+        p.dg = diag_at<C>(pc.tab, w.code, bx, pc.primary, pc.c).d;
         return p;
     }
     template <class X, class Acc>
@@ -448,7 +458,7 @@ struct PassOps<T, A, K_APPLY> {
         Pt<C> p;
         p.code = w.code;
         p.u = C(w.a);
-        p.dg = diag_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
+        p.dg = diag_at<C>(pc.tab, w.code, bx, pc.primary, pc.c).d;
         return p;
     }
     template <class X, class Acc>
@@ -479,8 +489,11 @@ struct PassOps<T, A, K_BICG1> {
         q.code = w.code;
         // p as it will be stored: the stencil and the x update must see the same direction
         const C p = C(T(C(w.a) + pc.beta * (C(w.b) - pc.omega * C(w.c))));
-        q.dg = diag_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
-        q.iv = inv_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
+        {
+            const DiagInv<C> di = diag_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
+            q.dg = di.d;
+            q.iv = di.iv;
+        }
         q.u = q.iv * p;
         q.aux = p;
         q.e = C(w.e);
@@ -515,8 +528,11 @@ struct PassOps<T, A, K_BICG2> {
         Pt<C> q;
         q.code = w.code;
         const C s = C(T(C(w.a) - pc.alpha * C(w.b)));
-        q.dg = diag_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
-        q.iv = inv_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
+        {
+            const DiagInv<C> di = diag_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
+            q.dg = di.d;
+            q.iv = di.iv;
+        }
         q.u = q.iv * s;
         q.aux = s;
         return q;
@@ -553,8 +569,11 @@ struct PassOps<T, A, K_BICG3> {
     __device__ static Pt<C> compute(const X& pc, const R& w, int bx, int k) {
         Pt<C> q;
         q.code = w.code;
-        q.dg = diag_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
-        q.iv = inv_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
+        {
+            const DiagInv<C> di = diag_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
+            q.dg = di.d;
+            q.iv = di.iv;
+        }
         q.u = q.iv * C(w.a);
         q.aux = C(w.a);
         q.e = C(w.e);
@@ -593,8 +612,11 @@ struct PassOps<T, A, K_CG1> {
     __device__ static Pt<C> compute(const X& pc, const R& w, int bx, int k) {
         Pt<C> q;
         q.code = w.code;
-        q.dg = diag_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
-        q.iv = inv_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
+        {
+            const DiagInv<C> di = diag_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
+            q.dg = di.d;
+            q.iv = di.iv;
+        }
         const C z = q.iv * C(w.a);
         // p as it will be stored: the stencil and the x update must see the same direction
         q.u = C(T(pc.first ? z : z + pc.beta * C(w.b)));
@@ -629,8 +651,11 @@ struct PassOps<T, A, K_CG2> {
     __device__ static Pt<C> compute(const X& pc, const R& w, int bx, int k) {
         Pt<C> q;
         q.code = w.code;
-        q.dg = diag_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
-        q.iv = inv_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
+        {
+            const DiagInv<C> di = diag_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
+            q.dg = di.d;
+            q.iv = di.iv;
+        }
         q.u = C(w.a);
         q.aux = C(w.b);
         q.e = C(w.c);
@@ -707,29 +732,30 @@ __device__ __forceinline__ void march(const Ctx& pc, const VolDesc& vd, int lt, 
 
     U u_prev = U(0), h_cur = U(0);
     Pt<U> c_cur{};
-    R rcA{}, rcB{}, rhA{}, rhB{};
-    if (z0 > 0) {
+    // Loads run kPrefetch planes ahead of their use: rc[j] / rh[j] hold the centre / halo
+    // loads of one plane, rotating through kPrefetch + 1 register sets (the loop is unrolled
+    // by that many planes, so a load in flight is never copied; a copy would wait for it).
+    constexpr int NB = kPrefetch + 1;
+    R rc[NB], rh[NB];
+    if (z0 > 0)
         u_prev = Ops::compute(pc, Ops::fetch(pc, clin + plane * (z0 - 1), false), bxz(cbxy, z0 - 1), kc(x, y, z0 - 1)).u;
-    }
-    {
-        c_cur = Ops::compute(pc, Ops::fetch(pc, clin + plane * z0, true), bxz(cbxy, z0), kc(x, y, z0));
-        rcA = Ops::fetch(pc, clin + plane * clampz(z0 + 1), true);
-    }
-    if (hwave) {
-        h_cur = Ops::compute(pc, Ops::fetch(pc, hlin + plane * z0, false), bxz(hbxy, z0), kc(hx, hy, z0)).u;
-        rhA = Ops::fetch(pc, hlin + plane * clampz(z0 + 1), false);
+    c_cur = Ops::compute(pc, Ops::fetch(pc, clin + plane * z0, true), bxz(cbxy, z0), kc(x, y, z0));
+    if (hwave) h_cur = Ops::compute(pc, Ops::fetch(pc, hlin + plane * z0, false), bxz(hbxy, z0), kc(hx, hy, z0)).u;
+#pragma unroll
+    for (int j = 0; j < kPrefetch; ++j) {
+        rc[j] = Ops::fetch(pc, clin + plane * clampz(z0 + 1 + j), true);
+        if (hwave) rh[j] = Ops::fetch(pc, hlin + plane * clampz(z0 + 1 + j), false);
     }
 
-    // One plane: `rc_in` / `rh_in` hold the loads of plane z+1 (issued one plane ago), the
-    // loads of plane z+2 go to `rc_out` / `rh_out`.  The loop below alternates the two
-    // register sets, so a load in flight is never copied (a copy would have to wait for it).
+    // One plane z: `rc_in` / `rh_in` hold the loads of plane z+1; the loads of plane
+    // z+1+kPrefetch go to `rc_out` / `rh_out`.
     auto step = [&](int z, const R& rc_in, R& rc_out, const R& rh_in, R& rh_out) {
         const bool has_next = (z + 1 < nz);
         const Pt<U> c_next = Ops::compute(pc, rc_in, bxz(cbxy, z + 1), kc(x, y, z + 1));
         U h_next = U(0);
         if (hwave) h_next = Ops::compute(pc, rh_in, bxz(hbxy, z + 1), kc(hx, hy, z + 1)).u;
-        rc_out = Ops::fetch(pc, clin + plane * clampz(z + 2), true);
-        if (hwave) rh_out = Ops::fetch(pc, hlin + plane * clampz(z + 2), false);
+        rc_out = Ops::fetch(pc, clin + plane * clampz(z + 1 + kPrefetch), true);
+        if (hwave) rh_out = Ops::fetch(pc, hlin + plane * clampz(z + 1 + kPrefetch), false);
         U* buf = lds + (z & 1) * PL;
         buf[(ty + 1) * W + (tx + 1)] = inside ? c_cur.u : U(0);
         if (hl >= 0) buf[hl] = hvalid ? h_cur : U(0);
@@ -745,9 +771,10 @@ __device__ __forceinline__ void march(const Ctx& pc, const VolDesc& vd, int lt, 
         c_cur = c_next;
         h_cur = h_next;
     };
-    for (int z = z0; z < z1; z += 2) {
-        step(z, rcA, rcB, rhA, rhB);
-        if (z + 1 < z1) step(z + 1, rcB, rcA, rhB, rhA);
+    for (int z = z0; z < z1; z += NB) {
+#pragma unroll
+        for (int u = 0; u < NB; ++u)
+            if (z + u < z1) step(z + u, rc[u], rc[(u + kPrefetch) % NB], rh[u], rh[(u + kPrefetch) % NB]);
     }
 }
 
@@ -1317,8 +1344,11 @@ struct MixOps {
         Pt<double> p;
         p.code = w.code;
         p.u = pc.first ? x0_of<double>(k, pc.np, pc.sgn) : w.a;
-        p.dg = diag_at<double>(pc.tab, w.code, bx, pc.primary, pc.c);
-        p.iv = inv_at<double>(pc.tab, w.code, bx, pc.primary, pc.c);
+        {
+            const DiagInv<double> di = diag_at<double>(pc.tab, w.code, bx, pc.primary, pc.c);
+            p.dg = di.d;
+            p.iv = di.iv;
+        }
         p.e = (double)w.e;
         return p;
     }
