@@ -126,6 +126,16 @@ def serving(ctx, T, W, n_req: int, loads=(0.5, 0.8, 1.1)) -> dict:
     return out
 
 
+def measured_traffic(applies: bool):
+    """HBM bytes per launch of the mixed PCG pass-2 kernel at 256^3, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE counters (copy-calibrated); None when not measured."""
+    path = ROOT / "profiles" / "cg2_mixed_256_traffic.txt"
+    if not applies or not path.exists():
+        return None
+    kv = dict(line.split("=", 1) for line in path.read_text().splitlines() if "=" in line and not line.startswith("#"))
+    return float(kv["traffic_bytes"])
+
+
 def profile_program(kind: str, n: int) -> int:
     """Short, traceable runs for rocprofv3 (kernel trace or counters)."""
     os.environ["FAST_TAU_GRAPHS"] = "0"   # every kernel launched directly
@@ -238,6 +248,7 @@ def main() -> int:
     }
     name, (kms, kbytes) = max(kernels.items(), key=lambda kv: kv[1][0])
     ach = kbytes / (kms * 1e-3) / 1e9
+    traffic = measured_traffic("K_CG2" in name and n == 256)
     others = {k: {"avg_launch_ms": v[0], "bytes_per_launch": v[1], "achieved_GBs": v[1] / (v[0] * 1e-3) / 1e9}
               for k, v in kernels.items()}
 
@@ -290,7 +301,8 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm", "kernel": name, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": ach / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": kbytes, "avg_launch_ms": kms,
+            "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "bytes_per_launch": kbytes, "avg_launch_ms": kms,
+            "timing": "HIP events on the solve stream around 20 back-to-back launches of the kernel",
         },
         "kernels_mixed_cg": others,
         "pass_ms": {"mixed_cg": mixed_pass, "fp64_cg": fp64_cg_pass, "fp64_bicgstab": fp64_bicg_pass},

@@ -972,6 +972,18 @@ static int time_passes(fast_volume* vol, int dir, int solver, int iters, double*
             }
         }
     }
+    // Kernel durations: each stencil pass launched `iters` times back to back between two
+    // events (no finalize in between, so every launch does the full pass), which keeps the
+    // event overhead out of the per-launch average.
+    for (int k = 0; k < npass && e == hipSuccess; ++k) {
+        e = hipEventRecord(ev[0], s);
+        for (int i = 0; i < iters && e == hipSuccess; ++i) e = Passes<T, A>::pass_only(B.L, solver, k, s);
+        if (e == hipSuccess) e = hipEventRecord(ev[1], s);
+        if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+        float t = 0;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, ev[0], ev[1]);
+        acc[k] = t;
+    }
     for (auto& x : ev) (void)hipEventDestroy(x);
     if (!hip_ok(ctx, e, "pass timing")) return FAST_E_HIP;
     for (int k = 0; k < 3; ++k) ms[k] = acc[k] / iters;

@@ -126,6 +126,9 @@ struct Passes {
     static hipError_t cg_iter(const Launch<T, A>&, hipStream_t);
     // Same, recording ev[2k] / ev[2k+1] around stencil pass k (k = 0, 1).
     static hipError_t cg_iter_timed(const Launch<T, A>&, hipStream_t, hipEvent_t* ev);
+    // Stencil kernel of pass k alone (k = 0, 1 for CG; 0, 1, 2 for BiCGSTAB), no finalize
+    // (timing: the volumes' scalars and statuses stay as they are).
+    static hipError_t pass_only(const Launch<T, A>&, int solver, int k, hipStream_t);
     // y = A x (diagnostic; same device code as the solver).
     static hipError_t apply(const Launch<T, A>&, const T* x, T* y, hipStream_t);
 };

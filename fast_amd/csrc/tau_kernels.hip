@@ -1290,6 +1290,25 @@ hipError_t Passes<T, A>::cg_iter_timed(const Launch<T, A>& L, hipStream_t s, hip
 }
 
 template <class T, class A>
+hipError_t Passes<T, A>::pass_only(const Launch<T, A>& L, int solver, int k, hipStream_t s) {
+#define FT_CALL(TXV, TYV)                                                                                  \
+    {                                                                                                      \
+        const dim3 g(L.ntiles), b(TXV * TYV);                                                              \
+        const T* n0 = nullptr;                                                                             \
+        T* n1 = nullptr;                                                                                   \
+        if (solver == 1 && k == 0) hipLaunchKernelGGL((k_stencil<T, A, TXV, TYV, K_CG1>), g, b, 0, s, L, n0, n1);   \
+        if (solver == 1 && k == 1) hipLaunchKernelGGL((k_stencil<T, A, TXV, TYV, K_CG2>), g, b, 0, s, L, n0, n1);   \
+        if (solver == 0 && k == 0) hipLaunchKernelGGL((k_stencil<T, A, TXV, TYV, K_BICG1>), g, b, 0, s, L, n0, n1); \
+        if (solver == 0 && k == 1) hipLaunchKernelGGL((k_stencil<T, A, TXV, TYV, K_BICG2>), g, b, 0, s, L, n0, n1); \
+        if (solver == 0 && k == 2) hipLaunchKernelGGL((k_stencil<T, A, TXV, TYV, K_BICG3>), g, b, 0, s, L, n0, n1); \
+        return hipGetLastError();                                                                          \
+    }
+    FT_SHAPES(FT_TRY)
+#undef FT_CALL
+    return hipErrorInvalidValue;
+}
+
+template <class T, class A>
 hipError_t Passes<T, A>::apply(const Launch<T, A>& L, const T* x, T* y, hipStream_t s) {
 #define FT_CALL(TXV, TYV)                                                                                        \
     hipLaunchKernelGGL((k_stencil<T, A, TXV, TYV, K_APPLY>), dim3(L.ntiles), dim3(TXV * TYV), 0, s, L, x, y);    \

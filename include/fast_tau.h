@@ -212,10 +212,11 @@ int fast_debug_apply_f64(fast_volume* vol, int dir, const double* x, double* y);
 int fast_debug_apply_f32(fast_volume* vol, int dir, const float* x, float* y);
 /* Right-hand side b, initial guess x0 and Jacobi inverse 1/diag(A) (FP64). */
 int fast_debug_system_f64(fast_volume* vol, int dir, double* b, double* x0, double* invdiag);
-/* Mean device time (ms, HIP events on the solve stream) per launch of each fused pass and
- * per iteration, over `iters` iterations of a solve set up on `vol` (precision FAST_FP64,
- * FAST_FP32, or FAST_MIXED for the FP32-storage / FP64-arithmetic passes).  ms: 4 doubles;
- * BiCGSTAB {pass1, pass2, pass3, iteration}, CG {stencil pass, update pass, 0, iteration}. */
+/* Device times (ms, HIP events on the solve stream) on a solve set up on `vol` (precision
+ * FAST_FP64, FAST_FP32, or FAST_MIXED for the FP32-storage / FP64-arithmetic passes): the mean
+ * duration of each stencil pass kernel, launched `iters` times back to back, and the mean
+ * period of a whole iteration (passes and per-volume finalizes) over `iters` iterations.
+ * ms: 4 doubles; BiCGSTAB {pass1, pass2, pass3, iteration}, CG {pass1, pass2, 0, iteration}. */
 int fast_debug_time_passes(fast_volume* vol, int dir, int precision, int solver, int iters, double* ms);
 
 #ifdef __cplusplus
