@@ -23,7 +23,7 @@ import numpy as np
 from . import tau as T
 from ._lib import lib, LIB_PATH
 
-SHAPES = [(64, 8), (64, 4), (32, 8), (32, 16), (16, 16)]
+SHAPES = [(64, 8), (64, 4), (64, 16), (32, 8), (32, 16), (32, 32), (16, 16)]
 TZS = [4, 8, 16, 32, 64]
 PRECISIONS = {0: 0, 2: 1}   # precision -> solver timed (FP64: BiCGSTAB, mixed: CG)
 

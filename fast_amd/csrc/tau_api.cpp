@@ -161,7 +161,8 @@ template Coef<float> make_coef<float>(int, int, int, int, double, double);
 // volume yields ~2048+ tiles.  Both depend only on the volume, which keeps per-volume
 // reductions independent of the batch.
 static bool shape_compiled(int tx, int ty) {
-    return (tx == 64 && (ty == 4 || ty == 8)) || (tx == 32 && (ty == 8 || ty == 16)) || (tx == 16 && ty == 16);
+    return (tx == 64 && (ty == 4 || ty == 8 || ty == 16)) || (tx == 32 && (ty == 8 || ty == 16 || ty == 32)) ||
+           (tx == 16 && ty == 16);
 }
 
 Shape choose_shape(int nx, int ny, int nz, int precision, const fast_ctx* ctx) {

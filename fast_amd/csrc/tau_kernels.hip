@@ -1150,7 +1150,7 @@ hipError_t launch_iter_cg(const Launch<T, A>& L, hipStream_t s, hipEvent_t* ev =
 }
 
 // Shape dispatch: the compiled block shapes.
-#define FT_SHAPES(X) X(64, 8) X(64, 4) X(32, 8) X(32, 16) X(16, 16)
+#define FT_SHAPES(X) X(64, 8) X(64, 4) X(64, 16) X(32, 8) X(32, 16) X(32, 32) X(16, 16)
 
 }  // namespace
 

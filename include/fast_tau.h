@@ -102,7 +102,8 @@ const char* fast_ctx_last_error(const fast_ctx* ctx);
 /* Bytes of device memory currently held by the context. */
 size_t fast_ctx_device_bytes(const fast_ctx* ctx);
 /* Force the tile shape of subsequent solves: a block of tx*ty threads owns a tx*ty column
- * brick and marches tz planes (tx,ty one of 64x8, 64x4, 32x8, 32x16, 16x16; tz 1..256).
+ * brick and marches tz planes (tx,ty one of 64x8, 64x4, 64x16, 32x8, 32x16, 32x32, 16x16;
+ * tz 1..256).
  * tx = 0 restores the automatic choice (the autotuned table, else a size rule).  Used by
  * the autotuner (fast_amd/autotune.py).  No reference counterpart. */
 int fast_ctx_set_tile_shape(fast_ctx* ctx, int tx, int ty, int tz);

@@ -11,6 +11,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 GOLD = Path(__file__).resolve().parent / "golden"
+BICG, CG = 0, 1
 
 
 def test_ccl_golden_cases(ctx):
@@ -56,3 +57,28 @@ def test_batch_equals_solo_bit_exact(ctx):
             solo, _ = v.solve(p)
             assert solo.status == 0 and b.status == 0
             assert solo.tau == b.tau and solo.iterations == b.iterations, (prec, solver, v.dims, solo.tau, b.tau)
+
+
+SHAPES = [(64, 8), (64, 4), (64, 16), (32, 8), (32, 16), (32, 32), (16, 16)]
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_every_tile_shape_gives_identical_results(ctx, precision):
+    """Dot products are rounded once from double-double sums, so the Krylov trajectory, tau
+    and iteration count do not depend on the tile shape (block shape and planes per tile)."""
+    from fast_amd._lib import lib
+    m = F.generate_spheres((70, 45, 38), 4.0, 0.45, 21)
+    vol = ctx.volume(m)
+    ref = {}
+    try:
+        for (tx, ty) in SHAPES:
+            for tz in (3, 16):
+                assert lib().fast_ctx_set_tile_shape(ctx.handle, tx, ty, tz) == 0
+                for solver in (BICG, CG):
+                    res, _ = vol.solve(T.make_params(2, solver, precision, 1e-7, 10000))
+                    got = (res.tau, res.iterations)
+                    ref.setdefault(solver, got)
+                    assert got == ref[solver], ((tx, ty, tz), solver, got, ref[solver])
+    finally:
+        lib().fast_ctx_set_tile_shape(ctx.handle, 0, 0, 0)
+        vol.close()
