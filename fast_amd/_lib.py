@@ -45,7 +45,8 @@ class TauParams(C.Structure):
         ("mixed_inner_tol", C.c_double),
         ("mixed_outer_tol", C.c_double),
         ("verbose", C.c_int),
-        ("reserved", C.c_int * 3),
+        ("mixed_full_grid", C.c_int),
+        ("reserved", C.c_int * 2),
     ]
 
 

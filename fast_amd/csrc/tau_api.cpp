@@ -427,8 +427,8 @@ struct Batch {
 
 // Run `enqueue` (one polling round) from a graph captured on first use.  The graph holds
 // this batch's kernel arguments, so it lives and dies with the batch.
-template <class T, class A, class F>
-static hipError_t replay(Batch<T, A>& B, F enqueue) {
+template <class BT, class F>
+static hipError_t replay(BT& B, F enqueue) {
     fast_ctx* ctx = B.ctx;
     if (!ctx->use_graphs) return enqueue();
     if (!B.gexec) {
@@ -593,12 +593,13 @@ static int prepare(Batch<T, A>& B) {
 // Poll the device-resident states every `chunk` iterations, keeping one chunk in flight
 // (no host bubble).  `step` enqueues one chunk.  Returns when every volume reached a final
 // status (`final_status`); restarts are served synchronously (they are rare).
-template <class T, class A, class Step, class Restart>
-static int drive(Batch<T, A>& B, int final_status, long max_chunks, Step step, Restart restart) {
+template <class BT, class Step, class Restart>
+static int drive(BT& B, int final_status, long max_chunks, Step step, Restart restart) {
+    using St = typename decltype(B.hst)::value_type;
     fast_ctx* ctx = B.ctx;
     const int nv = (int)B.jobs.size();
-    const size_t sbytes = sizeof(VolState<A>) * nv;
-    VolState<A>* snap[2] = {(VolState<A>*)pinned(ctx, 0, sbytes), (VolState<A>*)pinned(ctx, 1, sbytes)};
+    const size_t sbytes = sizeof(St) * nv;
+    St* snap[2] = {(St*)pinned(ctx, 0, sbytes), (St*)pinned(ctx, 1, sbytes)};
     if (!snap[0] || !snap[1]) {
         set_error(ctx, "pinned host allocation failed");
         return FAST_E_OOM;
@@ -619,7 +620,7 @@ static int drive(Batch<T, A>& B, int final_status, long max_chunks, Step step, R
             bool all_final = true, any_restart = false;
             if (ctx->verbose) {
                 for (int i = 0; i < nv; ++i) {
-                    const VolState<A>& q = snap[prev][i];
+                    const St& q = snap[prev][i];
                     std::fprintf(stderr, "fasttau: round %ld vol %d status %d iter %d rsq %.3e tol2 %.3e r64sq %.3e "
                                  "updates %d restarts %d rho %.3e alpha %.3e omega %.3e\n",
                                  chunks - 1, i, q.status, q.iter, (double)q.rsq, (double)q.tol2, (double)q.r64sq,
@@ -656,11 +657,11 @@ static int drive(Batch<T, A>& B, int final_status, long max_chunks, Step step, R
 
 // Flux integral on the device, then tau = porosity / (dc * N_primary) in T like the
 // reference (VolumeMeasures.cpp:266-275).
-template <class T, class A, class TC>
-static int finish(Batch<T, A>& B, const TC* conc, const Coef<TC>* coef_dev, const std::vector<bool>& ok) {
+template <class TC, class BT>
+static int finish(BT& B, const VolDesc* vol_dev, const TC* conc, const Coef<TC>* coef_dev, const std::vector<bool>& ok) {
     fast_ctx* ctx = B.ctx;
     const int nv = (int)B.jobs.size();
-    hipError_t e = ft::flux_planes<TC>(B.nplanes, B.L.vol, B.plane_vol, B.plane_k, coef_dev, conc, B.plane_out,
+    hipError_t e = ft::flux_planes<TC>(B.nplanes, vol_dev, B.plane_vol, B.plane_k, coef_dev, conc, B.plane_out,
                                        ctx->stream);
     std::vector<double> planes(B.nplanes);
     if (e == hipSuccess)
@@ -738,7 +739,7 @@ static int solve_plain(fast_ctx* ctx, std::vector<Job*>& jobs, Shape shape, int 
         j.res->refinements = 0;
         j.res->status = success ? FAST_SOLVED : FAST_NO_CONVERGENCE;
     }
-    return finish<T, A, T>(B, (const T*)B.L.vs.x, B.L.coef, ok);
+    return finish<T>(B, B.L.vol, (const T*)B.L.vs.x, B.L.coef, ok);
 }
 
 // Mixed form: FP32 BiCGSTAB with FP64 reductions and FP64 reliable updates.  After every
@@ -791,7 +792,248 @@ static int solve_mixed(fast_ctx* ctx, std::vector<Job*>& jobs, Shape shape, int 
         j.res->refinements = st.refinements;
         j.res->status = success ? FAST_SOLVED : FAST_NO_CONVERGENCE;
     }
-    return finish<float, double, double>(B, B.L.vs.x64, B.coef64, ok);
+    return finish<double>(B, B.L.vol, B.L.vs.x64, B.coef64, ok);
+}
+
+// Mixed form of CG on the red-black Schur complement (see ft::RbDesc).  One batch holds
+// compact per-colour vectors; the FP64 solution of both colours is scattered into a full-grid
+// vector for the flux integral, which aliases the FP32 vectors (dead by then).
+struct RbBatch {
+    fast_ctx* ctx = nullptr;
+    std::vector<Job*> jobs;
+    std::vector<RbDesc> hrb;
+    std::vector<VolDesc> hvol;
+    std::vector<Coef<double>> hcoef;
+    std::vector<VolState<double>> hst;
+    std::vector<int> htile, hplane, hplane_k;
+    int ntiles = 0, nplanes = 0;
+    void* meta = nullptr;
+    void* vecs = nullptr;
+    RbLaunch L{};
+    const VolDesc* dvol = nullptr;
+    double* plane_out = nullptr;
+    int* plane_vol = nullptr;
+    int* plane_k = nullptr;
+    hipGraphExec_t gexec = nullptr;
+
+    ~RbBatch() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (ctx) {
+            dfree(ctx, meta);
+            dfree(ctx, vecs);
+        }
+    }
+};
+
+// Planes per RB tile: enough tiles to fill the chip (about 2048), 4..32 planes each.
+static int rb_tz(int hx, int ny, int nz) {
+    if (const char* e = std::getenv("FAST_TAU_RB_TZ")) {
+        const int v = std::atoi(e);
+        if (v > 0) return v;
+    }
+    const long long txy = (long long)((hx + kRbTX - 1) / kRbTX) * ((ny + kRbTY - 1) / kRbTY);
+    return (int)std::max<long long>(4, std::min<long long>(32, (long long)nz * txy / 2048));
+}
+
+static int prepare_rb(RbBatch& B) {
+    fast_ctx* ctx = B.ctx;
+    const int nv = (int)B.jobs.size();
+    B.hrb.resize(nv);
+    B.hvol.resize(nv);
+    B.hcoef.resize(nv);
+    B.hst.assign(nv, VolState<double>{});
+    int tiles = 0, planes = 0;
+    size_t total = 0, total_c = 0;
+    for (int i = 0; i < nv; ++i) {
+        const Job& j = *B.jobs[i];
+        const fast_volume* v = j.vol;
+        if ((size_t)v->nx * v->ny * v->nz > kMaxSolveVoxels) {
+            set_error(ctx, "volume of %d x %d x %d voxels exceeds the solver's limit of %zu voxels", v->nx, v->ny, v->nz,
+                      kMaxSolveVoxels);
+            return FAST_E_BADARG;
+        }
+        RbDesc& r = B.hrb[i];
+        r.nx = v->nx;
+        r.ny = v->ny;
+        r.nz = v->nz;
+        r.hx = (v->nx + 1) / 2;
+        r.dir = j.prm.dir;
+        r.primary = j.prm.dir / 2;
+        r.sgn = (j.prm.dir % 2 == 0) ? 1 : -1;
+        r.tiles_x = (r.hx + kRbTX - 1) / kRbTX;
+        r.tiles_y = (v->ny + kRbTY - 1) / kRbTY;
+        r.tz = rb_tz(r.hx, v->ny, v->nz);
+        r.tiles_z = (v->nz + r.tz - 1) / r.tz;
+        r.tile_base = tiles;
+        r.ntiles = r.tiles_x * r.tiles_y * r.tiles_z;
+        r.off_c = (long long)total_c;
+        r.off = (long long)total;
+        r.code = v->d_code;
+        tiles += r.ntiles;
+        total_c += align_up((size_t)r.hx * v->ny * v->nz, 64);
+        // full-grid descriptor for the flux integral (only the geometry and offsets are read)
+        VolDesc& d = B.hvol[i];
+        d.nx = v->nx;
+        d.ny = v->ny;
+        d.nz = v->nz;
+        d.dir = r.dir;
+        d.primary = r.primary;
+        d.sgn = r.sgn;
+        d.tiles_x = d.tiles_y = d.tiles_z = d.tz = 1;
+        d.plane_base = planes;
+        d.off = r.off;
+        d.code = v->d_code;
+        const int np = d.primary == 0 ? d.nx : (d.primary == 1 ? d.ny : d.nz);
+        planes += np;
+        total += align_up((size_t)v->nx * v->ny * v->nz, 64);
+        B.hcoef[i] = make_coef<double>(v->nx, v->ny, v->nz, j.prm.dir, j.prm.d_pore, j.prm.d_solid);
+        VolState<double>& s = B.hst[i];
+        s.tol = j.prm.tolerance;
+        s.maxiter = j.prm.max_iter;
+        s.eps2 = 2.220446049250313e-16 * 2.220446049250313e-16;
+        s.mode = 1;
+        s.status = ST_FRESH;
+        s.otol = j.outer_tol;
+        s.delta2 = j.update_delta * j.update_delta;
+        s.restart_cos = ctx->mixed_restart_cos;
+        s.solver = FAST_CG;
+    }
+    B.ntiles = tiles;
+    B.nplanes = planes;
+    B.htile.resize(tiles);
+    B.hplane.resize(planes);
+    B.hplane_k.resize(planes);
+    for (int i = 0; i < nv; ++i) {
+        for (int t = 0; t < B.hrb[i].ntiles; ++t) B.htile[B.hrb[i].tile_base + t] = i;
+        const VolDesc& d = B.hvol[i];
+        const int np = d.primary == 0 ? d.nx : (d.primary == 1 ? d.ny : d.nz);
+        for (int k = 0; k < np; ++k) {
+            B.hplane[d.plane_base + k] = i;
+            B.hplane_k[d.plane_base + k] = k;
+        }
+    }
+    size_t o = 0;
+    const size_t o_rb = o;
+    o = align_up(o + sizeof(RbDesc) * nv, 256);
+    const size_t o_vol = o;
+    o = align_up(o + sizeof(VolDesc) * nv, 256);
+    const size_t o_coef = o;
+    o = align_up(o + sizeof(Coef<double>) * nv, 256);
+    const size_t o_st = o;
+    o = align_up(o + sizeof(VolState<double>) * nv, 256);
+    const size_t o_tile = o;
+    o = align_up(o + sizeof(int) * tiles, 256);
+    const size_t o_part = o;
+    o = align_up(o + sizeof(double) * 8 * (size_t)tiles, 256);
+    const size_t o_partb = o;
+    o = align_up(o + sizeof(double) * 8 * (size_t)tiles, 256);
+    const size_t o_pvol = o;
+    o = align_up(o + sizeof(int) * planes, 256);
+    const size_t o_pk = o;
+    o = align_up(o + sizeof(int) * planes, 256);
+    const size_t o_pout = o;
+    o = align_up(o + sizeof(double) * planes, 256);
+    const size_t o_tab = o;
+    o = align_up(o + sizeof(double) * 2 * kTabEntries * nv, 256);
+    const size_t meta_bytes = o;
+    // FP32 red vectors x, r, p[2], q (their region later holds the full-grid solution), then
+    // FP64 y (black), x64 (red), x64b (black)
+    const size_t f_bytes = align_up(std::max(5 * total_c * sizeof(float), total * sizeof(double)), 256);
+    const size_t vec_bytes = f_bytes + 3 * total_c * sizeof(double);
+    B.meta = dalloc(ctx, meta_bytes);
+    B.vecs = dalloc(ctx, vec_bytes);
+    if (!B.meta || !B.vecs) return FAST_E_OOM;
+    std::vector<char> h(meta_bytes, 0);
+    std::memcpy(h.data() + o_rb, B.hrb.data(), sizeof(RbDesc) * nv);
+    std::memcpy(h.data() + o_vol, B.hvol.data(), sizeof(VolDesc) * nv);
+    std::memcpy(h.data() + o_coef, B.hcoef.data(), sizeof(Coef<double>) * nv);
+    std::memcpy(h.data() + o_st, B.hst.data(), sizeof(VolState<double>) * nv);
+    std::memcpy(h.data() + o_tile, B.htile.data(), sizeof(int) * tiles);
+    std::memcpy(h.data() + o_pvol, B.hplane.data(), sizeof(int) * planes);
+    std::memcpy(h.data() + o_pk, B.hplane_k.data(), sizeof(int) * planes);
+    hipError_t e = hipMemcpyAsync(B.meta, h.data(), meta_bytes, hipMemcpyHostToDevice, ctx->stream);
+    // every element a pass reads is written first; the fill keeps the padding slots defined
+    if (e == hipSuccess) e = hipMemsetAsync(B.vecs, 0, vec_bytes, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);   // h goes out of scope
+    if (!hip_ok(ctx, e, "batch upload")) return FAST_E_HIP;
+    char* m = (char*)B.meta;
+    RbLaunch& L = B.L;
+    L.ntiles = tiles;
+    L.nvol = nv;
+    L.vol = (const RbDesc*)(m + o_rb);
+    L.tile_vol = (const int*)(m + o_tile);
+    L.coef = (const Coef<double>*)(m + o_coef);
+    L.st = (VolState<double>*)(m + o_st);
+    L.part = (double*)(m + o_part);
+    L.part_b = (double*)(m + o_partb);
+    L.tab = (double*)(m + o_tab);
+    B.dvol = (const VolDesc*)(m + o_vol);
+    B.plane_vol = (int*)(m + o_pvol);
+    B.plane_k = (int*)(m + o_pk);
+    B.plane_out = (double*)(m + o_pout);
+    float* fb = (float*)B.vecs;
+    L.x = fb + 0 * total_c;
+    L.r = fb + 1 * total_c;
+    L.p[0] = fb + 2 * total_c;
+    L.p[1] = fb + 3 * total_c;
+    L.q = fb + 4 * total_c;
+    L.xfull = (double*)B.vecs;
+    double* db = (double*)((char*)B.vecs + f_bytes);
+    L.y = db + 0 * total_c;
+    L.x64 = db + 1 * total_c;
+    L.x64b = db + 2 * total_c;
+    e = ft::rb_tables(L, ctx->stream);
+    return hip_ok(ctx, e, "diagonal tables") ? FAST_OK : FAST_E_HIP;
+}
+
+// CG with FP32 vectors and FP64 reliable updates on the Schur complement of the red voxels;
+// the same update rule and FP64 convergence test as solve_mixed.  The FP64 residual it tests
+// is that of the full system: the black equations hold exactly (x_b is eliminated in FP64),
+// so the full residual is the Schur residual on the red voxels.
+static int solve_mixed_rb(fast_ctx* ctx, std::vector<Job*>& jobs) {
+    RbBatch B;
+    B.ctx = ctx;
+    B.jobs = jobs;
+    int rc = prepare_rb(B);
+    if (rc) return rc;
+    hipStream_t s = ctx->stream;
+    hipError_t e = hipEventRecord(ctx->t0, s);
+    if (e == hipSuccess) e = ft::rb_update(B.L, s);
+    if (!hip_ok(ctx, e, "mixed init")) return FAST_E_HIP;
+    long maxit = 0;
+    for (auto* j : jobs) maxit = std::max(maxit, j->prm.max_iter);
+    const int chunk = ctx->chunk;
+    const long max_chunks = 3 * (maxit / chunk + 2) + 16;
+    auto round = [&]() -> hipError_t {
+        hipError_t e2 = hipSuccess;
+        for (int i = 0; i < chunk && e2 == hipSuccess; ++i) {
+            e2 = ft::rb_iter(B.L, s);
+            const bool upd = (i + 1) % kMixedUpdateEvery == 0 || i + 1 == chunk;
+            if (e2 == hipSuccess && upd) e2 = ft::rb_update(B.L, s);
+        }
+        return e2;
+    };
+    auto step = [&]() -> hipError_t { return replay(B, round); };
+    auto restart = [&]() -> hipError_t { return hipSuccess; };
+    rc = drive(B, ST_IDLE, max_chunks, step, restart);
+    if (rc) return rc;
+    e = ft::rb_finish(B.L, s);
+    if (!hip_ok(ctx, e, "scatter")) return FAST_E_HIP;
+    std::vector<bool> ok(jobs.size());
+    for (size_t i = 0; i < jobs.size(); ++i) {
+        const VolState<double>& st = B.hst[i];
+        Job& j = *jobs[i];
+        const double err = st.b64sq > 0 ? std::sqrt(st.r64sq / st.b64sq) : 0.0;
+        const bool success = err <= j.outer_tol;
+        ok[i] = success;
+        j.res->rel_residual = err;
+        j.res->true_rel_residual = err;
+        j.res->iterations = st.iter;
+        j.res->restarts = st.restarts;
+        j.res->refinements = st.refinements;
+        j.res->status = success ? FAST_SOLVED : FAST_NO_CONVERGENCE;
+    }
+    return finish<double>(B, B.dvol, B.L.xfull, B.L.coef, ok);
 }
 
 // Validate, fill defaults, group by (precision, solver, tile shape) and run each group as
@@ -819,11 +1061,19 @@ int solve_jobs(fast_ctx* ctx, std::vector<Job>& jobs) {
     }
     // one batched solve per (precision, solver); its block shape is the one chosen for the
     // largest volume (results do not depend on the tile shape), each volume keeps its own tz
-    std::map<std::pair<int, int>, std::vector<Job*>> groups;
-    for (auto& j : jobs) groups[{j.prm.precision, j.prm.solver}].push_back(&j);
+    std::map<std::tuple<int, int, int>, std::vector<Job*>> groups;
+    for (auto& j : jobs) {
+        const int rb = j.prm.precision == FAST_MIXED && j.prm.solver == FAST_CG && !j.prm.mixed_full_grid;
+        groups[{j.prm.precision, j.prm.solver, rb}].push_back(&j);
+    }
     int worst = FAST_OK;
     for (auto& g : groups) {
-        const int prec = g.first.first, solver = g.first.second;
+        const int prec = std::get<0>(g.first), solver = std::get<1>(g.first);
+        if (std::get<2>(g.first)) {
+            const int rc = solve_mixed_rb(ctx, g.second);
+            if (rc) return rc;
+            continue;
+        }
         const Job* big = g.second[0];
         for (const Job* j : g.second)
             if ((size_t)j->vol->nx * j->vol->ny * j->vol->nz > (size_t)big->vol->nx * big->vol->ny * big->vol->nz) big = j;

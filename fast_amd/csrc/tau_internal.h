@@ -144,6 +144,54 @@ struct Passes {
 hipError_t mixed_residual(const Launch<float, double>&, const Coef<double>* coef64, hipStream_t);
 hipError_t mixed_update(const Launch<float, double>&, hipStream_t);
 
+// ---- red-black reduced system (mixed form, CG) --------------------------------------------
+// The 7-point operator couples each voxel only to voxels of the other colour (colour =
+// (x + y + z) & 1).  Eliminating the black voxels leaves the Schur complement on the red ones,
+//   S x_r = b_S,  S p = D_r p - A_rb D_b^-1 A_br p,  b_S = b_r - A_rb D_b^-1 b_b,
+// and x_b = D_b^-1 (b_b - A_br x_r) afterwards.  Jacobi-preconditioned CG on S needs about
+// half the iterations of Jacobi-PCG on the full system, on half the unknowns.  Vectors of
+// one colour are stored compactly: row (y, z) holds the hx = ceil(nx / 2) voxels x = 2k + s,
+// s = (colour + y + z) & 1, at element k + hx (y + ny z).
+struct RbDesc {
+    int nx, ny, nz, hx;
+    int dir, primary, sgn, pad0;
+    int tiles_x, tiles_y, tiles_z, tz;   // tile grid over (k, y, z); tz = planes per tile
+    int tile_base, ntiles;
+    long long off_c;                     // element offset in the compact (per colour) vectors
+    long long off;                       // element offset in the full-grid solution
+    const uint8_t* code;                 // full-grid stencil code
+};
+
+struct RbLaunch {
+    int ntiles, nvol;
+    const RbDesc* vol;
+    const int* tile_vol;
+    const Coef<double>* coef;
+    VolState<double>* st;
+    double* part;     // ntiles * 8: red-node partial sums
+    double* part_b;   // ntiles * 8: black-node partial sums
+    double* tab;      // per volume 2 * kTabEntries: diagonals, inverses
+    float* x;         // red: FP32 correction since the last reliable update
+    float* r;         // red: residual
+    float* p[2];      // red: search direction (ping-pong by VolState::phase)
+    float* q;         // red: S p
+    double* y;        // black: D_b^-1 A_br p (FP64: S p cancels strongly)
+    double* x64;      // red: FP64 solution
+    double* x64b;     // black: FP64 solution recovered from x64
+    double* xfull;    // full grid: FP64 solution (for the flux integral)
+};
+
+constexpr int kRbTX = 64, kRbTY = 4;   // threads per block along k and y
+
+hipError_t rb_tables(const RbLaunch&, hipStream_t);
+// One CG iteration on S for the volumes in ST_RUNNING (4 passes, 2 per-volume finalizes).
+hipError_t rb_iter(const RbLaunch&, hipStream_t);
+// Reliable update / start for the volumes in ST_UPDATE or ST_FRESH: x64 += x, x = 0, then the
+// FP64 residual r = b_S - S x64 (x64 = x0 on the first step) and the FP64 convergence test.
+hipError_t rb_update(const RbLaunch&, hipStream_t);
+// Both colours scattered into the full-grid solution (the last update left x64b = x_b(x64)).
+hipError_t rb_finish(const RbLaunch&, hipStream_t);
+
 // Flux integral of the concentration `conc` (element offsets VolDesc::off) over a list of
 // planes: entry i is plane plane_k[i] (along the primary axis) of volume plane_vol[i];
 // plane_out[i] = (sum over that plane of D (c - c_next) / dx) / n, in T.

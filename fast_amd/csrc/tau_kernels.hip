@@ -1589,4 +1589,274 @@ template hipError_t flux_planes<double>(int, const VolDesc*, const int*, const i
 template hipError_t flux_planes<float>(int, const VolDesc*, const int*, const int*, const Coef<float>*, const float*,
                                        double*, hipStream_t);
 
+// ---- red-black reduced system (mixed form, CG) --------------------------------------------
+namespace {
+
+// Position of a thread of an RB tile: slot k of row (y, z) of one colour, z marching.
+struct RbPos {
+    int k, y, z0, z1;
+    bool row_ok;
+};
+
+__device__ __forceinline__ RbPos rb_pos(const RbDesc& d, int lt) {
+    const int tix = lt % d.tiles_x;
+    const int tiy = (lt / d.tiles_x) % d.tiles_y;
+    const int tiz = lt / (d.tiles_x * d.tiles_y);
+    RbPos p;
+    p.k = tix * kRbTX + (int)(threadIdx.x % kRbTX);
+    p.y = tiy * kRbTY + (int)(threadIdx.x / kRbTX);
+    p.z0 = tiz * d.tz;
+    p.z1 = min(p.z0 + d.tz, d.nz);
+    p.row_ok = (p.k < d.hx) && (p.y < d.ny);
+    return p;
+}
+
+// Per-volume LDS tables shared by the RB passes.
+struct RbTabs {
+    double tab[2 * kTabEntries];
+    FacePair<double> ftab[8];
+};
+
+__device__ __forceinline__ void rb_load_tables(const RbLaunch& L, int v, const Coef<double>& c, RbTabs& t) {
+    load_tables<double, kRbTX * kRbTY>(L.tab + (long long)v * 2 * kTabEntries, t.tab, t.ftab, c);
+}
+
+// Sum over the six faces of a voxel of c_face * u(neighbour of the other colour), the
+// neighbour values read from the compact vector `u` of the other colour (+0 outside the box).
+// x = 2k + s is the voxel; rows of the other colour hold voxel x at slot k, x + 1 at k + s,
+// x - 1 at k - 1 + s.
+template <class U>
+__device__ __forceinline__ double rb_gather(const U* u, const RbDesc& d, const FacePair<double>* ftab, int code,
+                                            int k, int s, int x, int y, int z) {
+    const unsigned hx = (unsigned)d.hx, row = (unsigned)(y + d.ny * z) * hx;
+    const unsigned plane = hx * (unsigned)d.ny;
+    const int o4 = (code >> 4) & 4;
+    const FacePair<double> px = ftab[(code & 3) | o4];
+    const FacePair<double> py = ftab[((code >> 2) & 3) | o4];
+    const FacePair<double> pz = ftab[((code >> 4) & 3) | o4];
+    const double uxp = (x + 1 < d.nx) ? (double)ld(u, row + k + s) : 0.0;
+    const double uxm = (x > 0) ? (double)ld(u, row + k - 1 + s) : 0.0;
+    const double uyp = (y + 1 < d.ny) ? (double)ld(u, row + hx + k) : 0.0;
+    const double uym = (y > 0) ? (double)ld(u, row - hx + k) : 0.0;
+    const double uzp = (z + 1 < d.nz) ? (double)ld(u, row + plane + k) : 0.0;
+    const double uzm = (z > 0) ? (double)ld(u, row - plane + k) : 0.0;
+    double acc = 0.0;
+    acc += pz.m * uzm;
+    acc += py.m * uym;
+    acc += px.m * uxm;
+    acc += px.p * uxp;
+    acc += py.p * uyp;
+    acc += pz.p * uzp;
+    return acc;
+}
+
+enum RbKind : int { RB_P = 0, RB_Y, RB_Q, RB_U, RB_UPD, RB_Y64, RB_R64, RB_SCATTER };
+
+// All RB passes: one thread per voxel of one colour and plane, marching tz planes.
+template <int KIND>
+__global__ __launch_bounds__(kRbTX* kRbTY) void k_rb(RbLaunch L) {
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
+    if (tile >= L.ntiles) return;
+    const int v = L.tile_vol[tile];
+    const RbDesc& d = L.vol[v];
+    VolState<double>* vs = L.st + v;
+    const int status = vs->status;
+    if (KIND == RB_UPD || KIND == RB_Y64 || KIND == RB_R64) {
+        if (status != ST_UPDATE && status != ST_FRESH) return;
+    } else if (KIND == RB_P || KIND == RB_Y || KIND == RB_Q || KIND == RB_U) {
+        if (status != ST_RUNNING) return;
+    }
+    constexpr bool kTables = (KIND != RB_UPD && KIND != RB_SCATTER);
+    __shared__ RbTabs tabs;
+    const Coef<double> c = L.coef[v];
+    if (kTables) rb_load_tables(L, v, c, tabs);
+    const RbPos ps = rb_pos(d, tile - d.tile_base);
+    constexpr bool kBlack = (KIND == RB_Y || KIND == RB_Y64);
+    const double alpha = vs->alpha, beta = vs->beta;
+    const bool first = vs->first != 0;
+    const bool first_res = vs->refinements == 0;   // x64 not yet set: use x0
+    const bool keep = vs->refinements > 0 && !vs->discard;
+    const int nprim = d.primary == 0 ? d.nx : (d.primary == 1 ? d.ny : d.nz);
+    const bool odd = (vs->phase & 1) != 0;
+    float* p_old = L.p[odd ? 1 : 0] + d.off_c;
+    float* p_new = L.p[odd ? 0 : 1] + d.off_c;
+    float* X = L.x + d.off_c;
+    float* Rr = L.r + d.off_c;
+    float* Q = L.q + d.off_c;
+    double* Y = L.y + d.off_c;
+    double* X64 = L.x64 + d.off_c;
+    double* X64B = L.x64b + d.off_c;
+    AccDD acc[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc[i].init();
+    if (ps.row_ok) {
+        const unsigned plane_c = (unsigned)d.hx * (unsigned)d.ny;
+        for (int z = ps.z0; z < ps.z1; ++z) {
+            const int s = ((kBlack ? 1 : 0) + ps.y + z) & 1;
+            const int x = 2 * ps.k + s;
+            const unsigned ci = (unsigned)ps.k + (unsigned)d.hx * (unsigned)ps.y + plane_c * (unsigned)z;
+            if (KIND == RB_SCATTER) {   // slot k of both colours: red x = 2k + s, black x = 2k + 1 - s
+                double* row = L.xfull + d.off + (long long)d.nx * ((long long)ps.y + (long long)d.ny * z);
+                if (x < d.nx) row[x] = X64[ci];
+                if (2 * ps.k + 1 - s < d.nx) row[2 * ps.k + 1 - s] = X64B[ci];
+                continue;
+            }
+            if (x >= d.nx) continue;
+            const unsigned lin = (unsigned)x + (unsigned)d.nx * ((unsigned)ps.y + (unsigned)d.ny * (unsigned)z);
+            if (KIND == RB_UPD) {
+                if (keep) X64[ci] = X64[ci] + (double)X[ci];
+                X[ci] = 0.0f;
+                continue;
+            }
+            const int code = ld(d.code, lin);
+            const int bx = box_flags(x, ps.y, z, d.nx, d.ny, d.nz);
+            const DiagInv<double> di = diag_at<double>(tabs.tab, code, bx, d.primary, c);
+            const int kp = d.primary == 0 ? x : (d.primary == 1 ? ps.y : z);
+            if (KIND == RB_P) {
+                const double z_ = di.iv * (double)ld(Rr, ci);
+                const double pn = first ? z_ : z_ + beta * (double)ld(p_old, ci);
+                st(p_new, ci, (float)pn);
+            } else if (KIND == RB_Y) {
+                const double g = rb_gather(p_new, d, tabs.ftab, code, ps.k, s, x, ps.y, z);
+                st(Y, ci, g / di.d);
+            } else if (KIND == RB_Q) {
+                const double pv = (double)ld(p_new, ci);
+                const double g = rb_gather(Y, d, tabs.ftab, code, ps.k, s, x, ps.y, z);
+                const double qv = di.d * pv - g;
+                st(Q, ci, (float)qv);
+                acc[0].add(pv, qv);
+            } else if (KIND == RB_U) {
+                const double pv = (double)ld(p_new, ci);
+                st(X, ci, (float)((double)ld(X, ci) + alpha * pv));
+                const float rs = (float)((double)ld(Rr, ci) - alpha * (double)ld(Q, ci));
+                st(Rr, ci, rs);
+                acc[0].add(rs, rs);
+                acc[1].add((double)rs, di.iv * (double)rs);
+            } else if (KIND == RB_Y64) {
+                // black: x_b = (b_b - sum c x_r) / D_b from the FP64 red solution (x0 before the first update)
+                double g;
+                if (KIND == RB_Y64 && first_res) {
+                    // red neighbours at x0: x0 depends only on the primary coordinate
+                    const int o4 = (code >> 4) & 4;
+                    const FacePair<double> px = tabs.ftab[(code & 3) | o4];
+                    const FacePair<double> py = tabs.ftab[((code >> 2) & 3) | o4];
+                    const FacePair<double> pz = tabs.ftab[((code >> 4) & 3) | o4];
+                    auto x0n = [&](int dxp, int dyp, int dzp, bool ok) {
+                        if (!ok) return 0.0;
+                        const int kk = d.primary == 0 ? x + dxp : (d.primary == 1 ? ps.y + dyp : z + dzp);
+                        return x0_of<double>(kk, nprim, d.sgn);
+                    };
+                    g = 0.0;
+                    g += pz.m * x0n(0, 0, -1, z > 0);
+                    g += py.m * x0n(0, -1, 0, ps.y > 0);
+                    g += px.m * x0n(-1, 0, 0, x > 0);
+                    g += px.p * x0n(1, 0, 0, x + 1 < d.nx);
+                    g += py.p * x0n(0, 1, 0, ps.y + 1 < d.ny);
+                    g += pz.p * x0n(0, 0, 1, z + 1 < d.nz);
+                } else {
+                    g = rb_gather(X64, d, tabs.ftab, code, ps.k, s, x, ps.y, z);
+                }
+                const double b = rhs_of<double>(code, kp, nprim, c);
+                st(X64B, ci, (b - g) / di.d);
+                acc[0].add(b, b);
+            } else if (KIND == RB_R64) {
+                const double xr = first_res ? x0_of<double>(kp, nprim, d.sgn) : ld(X64, ci);
+                if (first_res) st(X64, ci, xr);
+                const double b = rhs_of<double>(code, kp, nprim, c);
+                const double g = rb_gather(X64B, d, tabs.ftab, code, ps.k, s, x, ps.y, z);
+                const double r64 = (b - di.d * xr) - g;
+                const float r32 = (float)r64;
+                st(Rr, ci, r32);
+                acc[0].add(r64, r64);
+                acc[1].add(b, b);
+                acc[2].add(r32, r32);
+                acc[3].add((double)r32, di.iv * (double)r32);
+            }
+        }
+    }
+    if (KIND == RB_Q) block_reduce_store<AccDD, 1, kRbTX * kRbTY>(reinterpret_cast<AccDD(&)[1]>(acc), L.part + (long long)tile * kPart);
+    if (KIND == RB_U) block_reduce_store<AccDD, 2, kRbTX * kRbTY>(reinterpret_cast<AccDD(&)[2]>(acc), L.part + (long long)tile * kPart);
+    if (KIND == RB_Y64) block_reduce_store<AccDD, 1, kRbTX * kRbTY>(reinterpret_cast<AccDD(&)[1]>(acc), L.part_b + (long long)tile * kPart);
+    if (KIND == RB_R64) block_reduce_store<AccDD, 4, kRbTX * kRbTY>(acc, L.part + (long long)tile * kPart);
+}
+
+// Per-volume finalize of an RB pass (one block per volume, fixed-order reduction).
+template <int FIN>
+__global__ __launch_bounds__(kFinThreads) void k_rb_fin(RbLaunch L) {
+    const int v = blockIdx.x;
+    if (v >= L.nvol) return;
+    VolState<double>* s = L.st + v;
+    const RbDesc& d = L.vol[v];
+    if (FIN == F_MIXRES) {
+        if (s->status != ST_UPDATE && s->status != ST_FRESH) return;
+        double tot[4], totb[1];
+        reduce_volume<4, kFinThreads>(L.part, d.tile_base, d.ntiles, tot);
+        reduce_volume<1, kFinThreads>(L.part_b, d.tile_base, d.ntiles, totb);
+        if (threadIdx.x == 0) {
+            tot[1] += totb[0];   // |b|^2 over both colours
+            mixres_update(s, tot);
+        }
+        return;
+    }
+    if (s->status != ST_RUNNING) return;
+    constexpr int NP = (FIN == F_CG2) ? 2 : 1;
+    double tot[NP];
+    reduce_volume<NP, kFinThreads>(L.part, d.tile_base, d.ntiles, tot);
+    if (threadIdx.x == 0) {
+        double ta[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < NP; ++k) ta[k] = tot[k];
+        fin_update<double, FIN>(s, ta);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_rb_tables(const RbDesc* vol, const Coef<double>* coef, double* tab, int nvol) {
+    const int v = blockIdx.x;
+    if (v >= nvol) return;
+    const Coef<double> c = coef[v];
+    const int primary = vol[v].primary;
+    double* t = tab + (long long)v * 2 * kTabEntries;
+    for (int i = threadIdx.x; i < kTabEntries; i += 256) {
+        const double dg = diag_of<double>(i & 127, i >> 7, primary, c);
+        t[i] = dg;
+        t[kTabEntries + i] = 1.0 / dg;
+    }
+}
+
+template <int KIND>
+void rb_launch(const RbLaunch& L, hipStream_t s) {
+    hipLaunchKernelGGL((k_rb<KIND>), dim3(L.ntiles), dim3(kRbTX * kRbTY), 0, s, L);
+}
+
+}  // namespace
+
+hipError_t rb_tables(const RbLaunch& L, hipStream_t s) {
+    if (L.nvol <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_rb_tables, dim3(L.nvol), dim3(256), 0, s, L.vol, L.coef, L.tab, L.nvol);
+    return hipGetLastError();
+}
+
+hipError_t rb_iter(const RbLaunch& L, hipStream_t s) {
+    rb_launch<RB_P>(L, s);
+    rb_launch<RB_Y>(L, s);
+    rb_launch<RB_Q>(L, s);
+    hipLaunchKernelGGL((k_rb_fin<F_CG1>), dim3(L.nvol), dim3(kFinThreads), 0, s, L);
+    rb_launch<RB_U>(L, s);
+    hipLaunchKernelGGL((k_rb_fin<F_CG2>), dim3(L.nvol), dim3(kFinThreads), 0, s, L);
+    return hipGetLastError();
+}
+
+hipError_t rb_update(const RbLaunch& L, hipStream_t s) {
+    rb_launch<RB_UPD>(L, s);
+    rb_launch<RB_Y64>(L, s);
+    rb_launch<RB_R64>(L, s);
+    hipLaunchKernelGGL((k_rb_fin<F_MIXRES>), dim3(L.nvol), dim3(kFinThreads), 0, s, L);
+    return hipGetLastError();
+}
+
+hipError_t rb_finish(const RbLaunch& L, hipStream_t s) {
+    rb_launch<RB_SCATTER>(L, s);
+    return hipGetLastError();
+}
+
 }  // namespace ft

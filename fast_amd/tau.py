@@ -243,7 +243,7 @@ class Volume:
 
 def make_params(d=Dir.X_POS, solver=DiffusionSolverType.DSOLVER_BICGSTAB, precision="double", tol=1e-6,
                 max_iter=10000, coeffs=(1.0, 0.001), porosity=-1.0, inner_tol=0.0, outer_tol=0.0,
-                verbose=False) -> TauParams:
+                verbose=False, full_grid=False) -> TauParams:
     p = TauParams()
     lib().fast_tau_params_default(C.byref(p))
     p.dir = int(d)
@@ -257,6 +257,7 @@ def make_params(d=Dir.X_POS, solver=DiffusionSolverType.DSOLVER_BICGSTAB, precis
     p.mixed_inner_tol = float(inner_tol)
     p.mixed_outer_tol = float(outer_tol)
     p.verbose = int(bool(verbose))
+    p.mixed_full_grid = int(bool(full_grid))
     return p
 
 

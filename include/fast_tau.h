@@ -64,7 +64,9 @@ typedef struct fast_tau_params {
     double mixed_inner_tol; /* MIXED only: residual reduction between FP64 reliable updates (0 = 0.1) */
     double mixed_outer_tol; /* MIXED only: FP64 true relative residual target (0 = `tolerance`) */
     int verbose;
-    int reserved[3];
+    int mixed_full_grid;  /* MIXED + CG only: 0 (default) = CG on the red-black Schur complement
+                             (about half the iterations); 1 = CG on the full grid */
+    int reserved[2];
 } fast_tau_params;
 
 /* Output of one solve: getTortuosity's return value and iterationsOut, plus status. */

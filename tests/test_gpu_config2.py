@@ -56,19 +56,24 @@ def converged_tau(vol, d):
     return res.tau
 
 
-def test_mixed_cg_within_1e5_of_converged(config2):
+@pytest.mark.parametrize("full_grid", [False, True], ids=["reduced", "full_grid"])
+def test_mixed_cg_within_1e5_of_converged(config2, full_grid):
     """The mixed form (FP32 vectors, FP64 arithmetic and reliable updates) with CG at tol 1e-6
     lands within 1e-5 of the converged tau; so does FP64 CG.  (FP64 BiCGSTAB at tol 1e-6 is
-    up to ~1e-4 away on this volume, which is why the converged value is the yardstick.)"""
+    up to ~1e-4 away on this volume, which is why the converged value is the yardstick.)  On
+    the full grid it runs FP64 CG's recurrences, so its iteration count is within a few of the
+    oracle's; on the red-black Schur complement it needs at most 0.6x as many."""
     fx, digest, vol = config2
     for d in (0, 2, 4):
         conv = converged_tau(vol, d)
-        res, _ = vol.solve(T.make_params(d, 1, "mixed", 1e-6, 10000))
+        res, _ = vol.solve(T.make_params(d, 1, "mixed", 1e-6, 10000, full_grid=full_grid))
         err = abs(res.tau - conv) / conv
-        print(f"d={d}: mixed CG tau={res.tau!r} it={res.iterations} updates={res.refinements} "
+        print(f"d={d} full_grid={full_grid}: mixed CG tau={res.tau!r} it={res.iterations} updates={res.refinements} "
               f"true_res={res.true_rel_residual:.2e} err_vs_converged={err:.2e}")
         assert res.status == 0 and res.true_rel_residual <= 1e-6
         assert err <= 1e-5
         ref = fx.get(f"CG_{DIRNAMES[d]}")
-        if ref:   # same recurrences as FP64 CG: the iteration count may differ by a few
+        if ref and full_grid:   # same recurrences as FP64 CG: the iteration count may differ by a few
             assert abs(res.iterations - ref["iterations"]) <= max(5, ref["iterations"] // 50)
+        elif ref:
+            assert res.iterations <= 0.6 * ref["iterations"]

@@ -1,0 +1,118 @@
+"""Mixed CG on the red-black Schur complement (the default mixed CG) against the CPU oracle's
+converged solution and against mixed CG on the full grid.
+
+The reduced system eliminates the black voxels exactly (in FP64), so it solves the same linear
+system as the reference: its tau and concentration field converge to the same values, and its
+FP64 true residual is that of the full system.  What differs is the Krylov process (CG on S,
+about half the iterations), so the bars are accuracy bars, not bit-identity."""
+import numpy as np
+import pytest
+
+import fast_amd as F
+from fast_amd import tau as T
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+BICG, CG = 0, 1
+
+
+def rand_mask(shape, frac, seed):
+    rng = np.random.default_rng(seed)
+    return np.where(rng.random(shape) < frac, 255, 0).astype(np.uint8)
+
+
+# (nz, ny, nx): odd and even extents, one-voxel rows and planes, a volume narrower than a tile
+RAGGED = [
+    ("odd_37x23x19", lambda: rand_mask((19, 23, 37), 0.45, 1)),
+    ("even_40x30x20", lambda: F.generate_spheres((40, 30, 20), 3.0, 0.5, 2)),
+    ("tall_1x5x70", lambda: rand_mask((70, 5, 1), 0.3, 3)),
+    ("thin_67x3x1", lambda: rand_mask((1, 3, 67), 0.3, 4)),
+    ("wide_131x17x11", lambda: rand_mask((11, 17, 131), 0.4, 5)),
+    ("cube_9", lambda: rand_mask((9, 9, 9), 0.4, 6)),
+]
+
+
+@pytest.mark.parametrize("name,make", RAGGED, ids=[r[0] for r in RAGGED])
+def test_reduced_matches_converged_oracle_on_ragged_shapes(ctx, name, make):
+    """Every direction at outer tolerance 1e-11: tau within 1e-7 of the oracle's CG at 1e-13,
+    the concentration field within 1e-6 (absolute; it lies in [0, 1])."""
+    m = make()
+    vol = ctx.volume(m)
+    try:
+        for d in range(6):
+            ref, cref = O.tau(m, d, solver=CG, tol=1e-13, conc=True)
+            res, c = vol.solve(T.make_params(d, CG, "mixed", 1e-11, 20000), conc=True)
+            assert res.status == 0, (name, d, res.iterations)
+            assert res.true_rel_residual <= 1e-11
+            if np.isinf(ref.tau) or ref.tau == 0:
+                assert res.tau == ref.tau or (np.isinf(ref.tau) and np.isinf(res.tau)), (name, d, res.tau, ref.tau)
+                continue
+            err = abs(res.tau - ref.tau) / ref.tau
+            dc = np.abs(c - cref).max()
+            print(f"{name} d={d}: tau={res.tau!r} oracle={ref.tau!r} err={err:.2e} it={res.iterations} "
+                  f"(oracle {ref.iterations}) max|dc|={dc:.2e}")
+            assert err <= 1e-7, (name, d)
+            assert dc <= 1e-6, (name, d)
+    finally:
+        vol.close()
+
+
+def test_reduced_needs_about_half_the_iterations(ctx):
+    """64^3 spheres (radius 6, solid 0.60), X: CG on S converges in at most 0.6x the
+    iterations of CG on the full grid, to the same tau."""
+    m = F.generate_spheres(64, 6.0, 0.60, 0)
+    vol = ctx.volume(m)
+    tight, _ = O.tau(m, 0, solver=CG, tol=1e-12)
+    for tol in (1e-6, 1e-8):
+        rb, _ = vol.solve(T.make_params(0, CG, "mixed", tol, 10000))
+        full, _ = vol.solve(T.make_params(0, CG, "mixed", tol, 10000, full_grid=True))
+        print(f"tol {tol:g}: reduced it={rb.iterations} updates={rb.refinements} err="
+              f"{abs(rb.tau - tight.tau) / tight.tau:.2e}; full it={full.iterations} err="
+              f"{abs(full.tau - tight.tau) / tight.tau:.2e}")
+        assert rb.status == 0 and full.status == 0
+        assert rb.iterations <= 0.6 * full.iterations
+        assert abs(rb.tau - tight.tau) / tight.tau <= (3e-5 if tol == 1e-6 else 5e-7)
+    vol.close()
+
+
+def test_reduced_closed_forms(ctx):
+    """All pore: tau = 1 in every direction.  Parallel slabs along X: tau =
+    float(phi) / (phi + 0.001f (1 - phi)) (slabs in parallel: the arithmetic mean)."""
+    m = np.zeros((21, 16, 33), np.uint8)
+    vol = ctx.volume(m)
+    for d in range(6):
+        res, _ = vol.solve(T.make_params(d, CG, "mixed", 1e-10, 10000))
+        assert res.status == 0 and abs(res.tau - 1.0) < 1e-9, (d, res.tau)
+    vol.close()
+    m = np.zeros((30, 27, 45), np.uint8)
+    m[:, ::3, :] = 255
+    phi = (m == 0).mean()
+    expected = float(np.float32(phi)) / (phi + float(np.float32(0.001)) * (1 - phi))
+    vol = ctx.volume(m)
+    res, _ = vol.solve(T.make_params(0, CG, "mixed", 1e-11, 10000))
+    assert res.status == 0 and abs(res.tau - expected) < 1e-8 * expected, (res.tau, expected)
+    vol.close()
+
+
+def test_reduced_max_iter_reports_no_convergence(ctx):
+    m = F.generate_spheres(64, 6.0, 0.60, 0)
+    vol = ctx.volume(m)
+    res, _ = vol.solve(T.make_params(0, CG, "mixed", 1e-8, 5))
+    assert res.status == 1 and res.tau == 0.0 and res.iterations == 5
+    vol.close()
+
+
+def test_reduced_batch_equals_solo_uneven(ctx):
+    """Volumes of different sizes and directions in one batch: each result is bit-identical
+    to its solo solve (per-volume dot products reduce in a fixed order)."""
+    shapes = [(33, 20, 17), (64, 64, 64), (9, 40, 31), (50, 21, 48)]
+    vols, ps = [], []
+    for i, sh in enumerate(shapes):
+        vols.append(ctx.volume(F.generate_spheres(sh, 3.0 + i, 0.45, 40 + i)))
+        ps.append(T.make_params(i % 6, CG, "mixed", 1e-8, 10000))
+    batch = T.solve_batch(ctx, vols, ps)
+    for v, p, b in zip(vols, ps, batch):
+        solo, _ = v.solve(p)
+        assert b.status == 0 and solo.status == 0
+        assert b.tau == solo.tau and b.iterations == solo.iterations and b.refinements == solo.refinements
+        v.close()

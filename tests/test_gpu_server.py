@@ -1,5 +1,6 @@
 """Continuous-batching server: every request's result is bit-identical to solving its volume
-alone in the mixed form, whatever else was in flight, and admission respects the HBM budget."""
+alone in the mixed form on the full grid, whatever else was in flight, and admission respects
+the HBM budget."""
 import numpy as np
 import pytest
 
@@ -27,7 +28,7 @@ def test_server_matches_solo(ctx, solver):
     solo = []
     for m, d in reqs:
         vol = ctx.volume(m)
-        res, _ = vol.solve(T.make_params(d, solver, "mixed", 1e-7, 10000))
+        res, _ = vol.solve(T.make_params(d, solver, "mixed", 1e-7, 10000, full_grid=True))
         vol.close()
         solo.append(res)
     # few slots and short rounds: requests join and leave while others are running
