@@ -152,9 +152,12 @@ def profile_program(kind: str, n: int) -> int:
             print(f"{name}: tau {res.tau!r} iterations {res.iterations} {res.solve_ms:.1f} ms", flush=True)
     else:
         ms = (C.c_double * 4)()
-        for name, prec, solver in (("fp64 bicgstab", 0, BICG), ("fp64 cg", 0, CG), ("mixed cg", 2, CG)):
+        for name, prec, solver in (("fp64 bicgstab", 0, BICG), ("fp64 cg", 0, CG), ("mixed cg full grid", 2, CG)):
             lib().fast_debug_time_passes(vol.handle, 0, prec, solver, 20, ms)
             print(f"{name}: pass ms {list(ms)[:3]} iteration ms {ms[3]}", flush=True)
+        rb = (C.c_double * 5)()
+        lib().fast_debug_time_rb_passes(vol.handle, 0, 20, rb)
+        print(f"mixed cg red-black: P, Y, Q, U ms {list(rb)[:4]} iteration ms {rb[4]}", flush=True)
         import torch   # calibration kernel of known traffic: read 256 MiB, write 256 MiB
         a = torch.ones(64 * 2 ** 20, dtype=torch.float32, device="cuda")
         b = torch.empty_like(a)

@@ -118,6 +118,7 @@ PROTOTYPES = [
     ("fast_debug_apply_f32", C.c_int, [_vp, C.c_int, _fp, _fp]),
     ("fast_debug_system_f64", C.c_int, [_vp, C.c_int, _dp, _dp, _dp]),
     ("fast_debug_time_passes", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
+    ("fast_debug_time_rb_passes", C.c_int, [_vp, C.c_int, C.c_int, _dp]),
 ]
 
 _lib = None

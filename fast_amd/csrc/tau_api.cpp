@@ -857,6 +857,8 @@ static int prepare_rb(RbBatch& B) {
         r.ny = v->ny;
         r.nz = v->nz;
         r.hx = (v->nx + 1) / 2;
+        r.px = r.hx + 2;
+        r.pp = (unsigned)r.px * (unsigned)(v->ny + 2);
         r.dir = j.prm.dir;
         r.primary = j.prm.dir / 2;
         r.sgn = (j.prm.dir % 2 == 0) ? 1 : -1;
@@ -870,7 +872,7 @@ static int prepare_rb(RbBatch& B) {
         r.off = (long long)total;
         r.code = v->d_code;
         tiles += r.ntiles;
-        total_c += align_up((size_t)r.hx * v->ny * v->nz, 64);
+        total_c += align_up((size_t)r.pp * (size_t)(v->nz + 2), 64);
         // full-grid descriptor for the flux integral (only the geometry and offsets are read)
         VolDesc& d = B.hvol[i];
         d.nx = v->nx;
@@ -937,9 +939,10 @@ static int prepare_rb(RbBatch& B) {
     o = align_up(o + sizeof(double) * 2 * kTabEntries * nv, 256);
     const size_t meta_bytes = o;
     // FP32 red vectors x, r, p[2], q (their region later holds the full-grid solution), then
-    // FP64 y (black), x64 (red), x64b (black)
+    // FP64 y (black), x64 (red), x64b (black), then the per-colour codes
     const size_t f_bytes = align_up(std::max(5 * total_c * sizeof(float), total * sizeof(double)), 256);
-    const size_t vec_bytes = f_bytes + 3 * total_c * sizeof(double);
+    const size_t d_bytes = 3 * total_c * sizeof(double);
+    const size_t vec_bytes = f_bytes + d_bytes + 2 * total_c;
     B.meta = dalloc(ctx, meta_bytes);
     B.vecs = dalloc(ctx, vec_bytes);
     if (!B.meta || !B.vecs) return FAST_E_OOM;
@@ -952,7 +955,7 @@ static int prepare_rb(RbBatch& B) {
     std::memcpy(h.data() + o_pvol, B.hplane.data(), sizeof(int) * planes);
     std::memcpy(h.data() + o_pk, B.hplane_k.data(), sizeof(int) * planes);
     hipError_t e = hipMemcpyAsync(B.meta, h.data(), meta_bytes, hipMemcpyHostToDevice, ctx->stream);
-    // every element a pass reads is written first; the fill keeps the padding slots defined
+    // ghost and padding elements must read +0 (they are never written)
     if (e == hipSuccess) e = hipMemsetAsync(B.vecs, 0, vec_bytes, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);   // h goes out of scope
     if (!hip_ok(ctx, e, "batch upload")) return FAST_E_HIP;
@@ -960,6 +963,8 @@ static int prepare_rb(RbBatch& B) {
     RbLaunch& L = B.L;
     L.ntiles = tiles;
     L.nvol = nv;
+    const char* stg = std::getenv("FAST_TAU_RB_STAGED");
+    L.staged = !(stg && stg[0] == '0');
     L.vol = (const RbDesc*)(m + o_rb);
     L.tile_vol = (const int*)(m + o_tile);
     L.coef = (const Coef<double>*)(m + o_coef);
@@ -982,7 +987,14 @@ static int prepare_rb(RbBatch& B) {
     L.y = db + 0 * total_c;
     L.x64 = db + 1 * total_c;
     L.x64b = db + 2 * total_c;
-    e = ft::rb_tables(L, ctx->stream);
+    uint8_t* codes = (uint8_t*)B.vecs + f_bytes + d_bytes;
+    for (int i = 0; i < nv; ++i) {
+        B.hrb[i].code_c[0] = codes + B.hrb[i].off_c;
+        B.hrb[i].code_c[1] = codes + total_c + B.hrb[i].off_c;
+    }
+    e = hipMemcpyAsync(m + o_rb, B.hrb.data(), sizeof(RbDesc) * nv, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = ft::rb_setup(L, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);   // hrb is read by the copy
     return hip_ok(ctx, e, "diagonal tables") ? FAST_OK : FAST_E_HIP;
 }
 
@@ -1242,6 +1254,49 @@ static int time_passes(fast_volume* vol, int dir, int solver, int iters, double*
     return FAST_OK;
 }
 
+// Red-black passes P, Y, Q, U: each launched `iters` times back to back between two events
+// (ms[0..3], average per launch), then `iters` whole iterations (ms[4], average period).  The
+// solve is held in its Krylov phase (no reliable update) so that every launch does full work.
+static int time_rb_passes(fast_volume* vol, int dir, int iters, double* ms) {
+    fast_ctx* ctx = vol->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    Job j;
+    j.vol = vol;
+    fast_tau_params_default(&j.prm);
+    j.prm.dir = dir;
+    j.prm.solver = FAST_CG;
+    j.prm.precision = FAST_MIXED;
+    j.prm.max_iter = 1L << 40;
+    j.outer_tol = 1e-300;
+    j.update_delta = 1e-150;
+    fast_tau_result res;
+    j.res = &res;
+    RbBatch B;
+    B.ctx = ctx;
+    B.jobs = {&j};
+    int rc = prepare_rb(B);
+    if (rc) return rc;
+    hipStream_t s = ctx->stream;
+    hipEvent_t ev[2];
+    (void)hipEventCreate(&ev[0]);
+    (void)hipEventCreate(&ev[1]);
+    hipError_t e = ft::rb_update(B.L, s);
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) e = ft::rb_iter(B.L, s);
+    for (int k = 0; k < 5 && e == hipSuccess; ++k) {
+        e = hipEventRecord(ev[0], s);
+        for (int i = 0; i < iters && e == hipSuccess; ++i) e = k == 4 ? ft::rb_iter(B.L, s) : ft::rb_pass_only(B.L, k, s);
+        if (e == hipSuccess) e = hipEventRecord(ev[1], s);
+        if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+        float t = 0;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, ev[0], ev[1]);
+        ms[k] = t / iters;
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    return hip_ok(ctx, e, "pass timing") ? FAST_OK : FAST_E_HIP;
+}
+
 extern "C" {
 
 int fast_debug_time_passes(fast_volume* vol, int dir, int precision, int solver, int iters, double* ms) {
@@ -1251,6 +1306,11 @@ int fast_debug_time_passes(fast_volume* vol, int dir, int precision, int solver,
     if (precision == FAST_MIXED) return time_passes<float, double>(vol, dir, solver, iters, ms);
     if (precision == FAST_FP32) return time_passes<float, float>(vol, dir, solver, iters, ms);
     return FAST_E_BADARG;
+}
+
+int fast_debug_time_rb_passes(fast_volume* vol, int dir, int iters, double* ms) {
+    if (!vol || !ms || iters < 1 || dir < 0 || dir > 5) return FAST_E_BADARG;
+    return time_rb_passes(vol, dir, iters, ms);
 }
 
 int fast_debug_apply_f64(fast_volume* vol, int dir, const double* x, double* y) { return debug_apply<double>(vol, dir, x, y); }

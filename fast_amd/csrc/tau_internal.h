@@ -149,27 +149,32 @@ hipError_t mixed_update(const Launch<float, double>&, hipStream_t);
 // (x + y + z) & 1).  Eliminating the black voxels leaves the Schur complement on the red ones,
 //   S x_r = b_S,  S p = D_r p - A_rb D_b^-1 A_br p,  b_S = b_r - A_rb D_b^-1 b_b,
 // and x_b = D_b^-1 (b_b - A_br x_r) afterwards.  Jacobi-preconditioned CG on S needs about
-// half the iterations of Jacobi-PCG on the full system, on half the unknowns.  Vectors of
-// one colour are stored compactly: row (y, z) holds the hx = ceil(nx / 2) voxels x = 2k + s,
-// s = (colour + y + z) & 1, at element k + hx (y + ny z).
+// half the iterations of Jacobi-PCG on the full system, on half the unknowns.  Vectors of one
+// colour are stored compactly with a ghost layer: row (y, z) holds the hx = ceil(nx / 2)
+// voxels x = 2k + s, s = (colour + y + z) & 1, at element (k + 1) + px (y + 1) + pp (z + 1),
+// px = hx + 2, pp = px (ny + 2).  Ghost elements and slots with x >= nx hold 0 and are never
+// written, so neighbour reads need no bounds tests.
 struct RbDesc {
     int nx, ny, nz, hx;
-    int dir, primary, sgn, pad0;
+    int dir, primary, sgn, px;           // px: row pitch of the padded layout
     int tiles_x, tiles_y, tiles_z, tz;   // tile grid over (k, y, z); tz = planes per tile
     int tile_base, ntiles;
-    long long off_c;                     // element offset in the compact (per colour) vectors
+    unsigned pp, pad0;                   // pp: plane pitch of the padded layout
+    long long off_c;                     // element offset in the per-colour vectors
     long long off;                       // element offset in the full-grid solution
     const uint8_t* code;                 // full-grid stencil code
+    const uint8_t* code_c[2];            // per-colour stencil codes (padded layout; red, black)
 };
 
 struct RbLaunch {
     int ntiles, nvol;
+    int staged;       // 1: Q pass with the Y planes staged in LDS; 0: six Y loads per voxel
     const RbDesc* vol;
     const int* tile_vol;
     const Coef<double>* coef;
     VolState<double>* st;
-    double* part;     // ntiles * 8: red-node partial sums
-    double* part_b;   // ntiles * 8: black-node partial sums
+    double* part;     // ntiles * 8: red-voxel partial sums
+    double* part_b;   // ntiles * 8: black-voxel partial sums
     double* tab;      // per volume 2 * kTabEntries: diagonals, inverses
     float* x;         // red: FP32 correction since the last reliable update
     float* r;         // red: residual
@@ -183,12 +188,15 @@ struct RbLaunch {
 
 constexpr int kRbTX = 64, kRbTY = 4;   // threads per block along k and y
 
-hipError_t rb_tables(const RbLaunch&, hipStream_t);
+// Diagonal tables and per-colour codes (after the vectors and codes were zero-filled).
+hipError_t rb_setup(const RbLaunch&, hipStream_t);
 // One CG iteration on S for the volumes in ST_RUNNING (4 passes, 2 per-volume finalizes).
 hipError_t rb_iter(const RbLaunch&, hipStream_t);
 // Reliable update / start for the volumes in ST_UPDATE or ST_FRESH: x64 += x, x = 0, then the
 // FP64 residual r = b_S - S x64 (x64 = x0 on the first step) and the FP64 convergence test.
 hipError_t rb_update(const RbLaunch&, hipStream_t);
+// One pass of rb_iter alone (0 = P, 1 = Y, 2 = Q, 3 = U), for timing.
+hipError_t rb_pass_only(const RbLaunch&, int k, hipStream_t);
 // Both colours scattered into the full-grid solution (the last update left x64b = x_b(x64)).
 hipError_t rb_finish(const RbLaunch&, hipStream_t);
 

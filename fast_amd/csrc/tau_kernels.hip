@@ -29,6 +29,7 @@
 // and of the batch, and matches a sequential CPU evaluation of the same recurrences bit for
 // bit.
 #include "tau_internal.h"
+
 #include <cfloat>
 
 namespace ft {
@@ -1592,6 +1593,12 @@ template hipError_t flux_planes<float>(int, const VolDesc*, const int*, const in
 // ---- red-black reduced system (mixed form, CG) --------------------------------------------
 namespace {
 
+// Element of slot k of row (y, z) in the padded per-colour layout (a ghost layer around the
+// box; ghost slots and the slots past the end of odd rows hold 0 and are never written).
+__device__ __forceinline__ unsigned rb_index(const RbDesc& d, int k, int y, int z) {
+    return (unsigned)(k + 1) + (unsigned)d.px * (unsigned)(y + 1) + d.pp * (unsigned)(z + 1);
+}
+
 // Position of a thread of an RB tile: slot k of row (y, z) of one colour, z marching.
 struct RbPos {
     int k, y, z0, z1;
@@ -1621,122 +1628,331 @@ __device__ __forceinline__ void rb_load_tables(const RbLaunch& L, int v, const C
     load_tables<double, kRbTX * kRbTY>(L.tab + (long long)v * 2 * kTabEntries, t.tab, t.ftab, c);
 }
 
-// Sum over the six faces of a voxel of c_face * u(neighbour of the other colour), the
-// neighbour values read from the compact vector `u` of the other colour (+0 outside the box).
-// x = 2k + s is the voxel; rows of the other colour hold voxel x at slot k, x + 1 at k + s,
-// x - 1 at k - 1 + s.
+// Loads of the six neighbours of the other colour, z-, y-, x-, x+, y+, z+, of the voxel
+// x = 2k + s at element ci.  Rows of the other colour hold voxel x at slot k, x + 1 at slot
+// k + s and x - 1 at slot k - 1 + s; a neighbour outside the box is a ghost (+0).
 template <class U>
-__device__ __forceinline__ double rb_gather(const U* u, const RbDesc& d, const FacePair<double>* ftab, int code,
-                                            int k, int s, int x, int y, int z) {
-    const unsigned hx = (unsigned)d.hx, row = (unsigned)(y + d.ny * z) * hx;
-    const unsigned plane = hx * (unsigned)d.ny;
+__device__ __forceinline__ void rb_nbrs(const U* u, const RbDesc& d, unsigned ci, int s, U (&o)[6]) {
+    o[0] = ld(u, ci - d.pp);
+    o[1] = ld(u, ci - (unsigned)d.px);
+    o[2] = ld(u, ci - 1u + (unsigned)s);
+    o[3] = ld(u, ci + (unsigned)s);
+    o[4] = ld(u, ci + (unsigned)d.px);
+    o[5] = ld(u, ci + d.pp);
+}
+
+// Sum over the faces of c_face * (neighbour value), in the order of the row product
+// (Z-, Y-, X-, X+, Y+, Z+); ghost terms are c * (+0) and leave the sum unchanged.
+template <class U>
+__device__ __forceinline__ double rb_sum(const FacePair<double>* ftab, int code, const U (&o)[6]) {
     const int o4 = (code >> 4) & 4;
     const FacePair<double> px = ftab[(code & 3) | o4];
     const FacePair<double> py = ftab[((code >> 2) & 3) | o4];
     const FacePair<double> pz = ftab[((code >> 4) & 3) | o4];
-    const double uxp = (x + 1 < d.nx) ? (double)ld(u, row + k + s) : 0.0;
-    const double uxm = (x > 0) ? (double)ld(u, row + k - 1 + s) : 0.0;
-    const double uyp = (y + 1 < d.ny) ? (double)ld(u, row + hx + k) : 0.0;
-    const double uym = (y > 0) ? (double)ld(u, row - hx + k) : 0.0;
-    const double uzp = (z + 1 < d.nz) ? (double)ld(u, row + plane + k) : 0.0;
-    const double uzm = (z > 0) ? (double)ld(u, row - plane + k) : 0.0;
     double acc = 0.0;
-    acc += pz.m * uzm;
-    acc += py.m * uym;
-    acc += px.m * uxm;
-    acc += px.p * uxp;
-    acc += py.p * uyp;
-    acc += pz.p * uzp;
+    acc += pz.m * (double)o[0];
+    acc += py.m * (double)o[1];
+    acc += px.m * (double)o[2];
+    acc += px.p * (double)o[3];
+    acc += py.p * (double)o[4];
+    acc += pz.p * (double)o[5];
     return acc;
 }
 
-enum RbKind : int { RB_P = 0, RB_Y, RB_Q, RB_U, RB_UPD, RB_Y64, RB_R64, RB_SCATTER };
+enum RbKind : int { RB_P = 0, RB_Y, RB_Q, RB_U, RB_UPD, RB_Y64, RB_R64, RB_SCATTER, RB_CODE };
 
-// All RB passes: one thread per voxel of one colour and plane, marching tz planes.
+// Loaded inputs of one voxel of an iteration pass.
+struct RbIn {
+    int code;
+    float a, b, c, e;   // centre values
+    float u[6];         // Y: red neighbours (p)
+    double w[6];        // Q: black neighbours (Y)
+};
+
+// The iteration passes, one thread per slot of a row, marching tz planes:
+//   P (red):   p = D^-1 r (first iteration) or D^-1 r + beta p_old
+//   Y (black): Y = D_b^-1 (A_br p)
+//   Q (red):   q = D_r p - A_rb Y = S p; partial p.q
+//   U (red):   x += alpha p; r -= alpha q; partials r.r and r.(D^-1 r)
+// Loads are issued two planes ahead of use into three register sets used in rotation (the loop
+// is unrolled by 3), all unconditional: an invalid point reads the first interior slot, and
+// ghosts make neighbour reads safe, so each plane waits only for its own loads.
 template <int KIND>
-__global__ __launch_bounds__(kRbTX* kRbTY) void k_rb(RbLaunch L) {
+__global__ __launch_bounds__(kRbTX* kRbTY) void k_rb_it(RbLaunch L) {
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
     if (tile >= L.ntiles) return;
     const int v = L.tile_vol[tile];
-    const RbDesc& d = L.vol[v];
-    VolState<double>* vs = L.st + v;
-    const int status = vs->status;
-    if (KIND == RB_UPD || KIND == RB_Y64 || KIND == RB_R64) {
-        if (status != ST_UPDATE && status != ST_FRESH) return;
-    } else if (KIND == RB_P || KIND == RB_Y || KIND == RB_Q || KIND == RB_U) {
-        if (status != ST_RUNNING) return;
-    }
-    constexpr bool kTables = (KIND != RB_UPD && KIND != RB_SCATTER);
+    const RbDesc d = L.vol[v];   // by value: kept in registers across the stores
+    const VolState<double>* vs = L.st + v;
+    if (vs->status != ST_RUNNING) return;
     __shared__ RbTabs tabs;
     const Coef<double> c = L.coef[v];
-    if (kTables) rb_load_tables(L, v, c, tabs);
+    rb_load_tables(L, v, c, tabs);
     const RbPos ps = rb_pos(d, tile - d.tile_base);
-    constexpr bool kBlack = (KIND == RB_Y || KIND == RB_Y64);
+    constexpr int kColour = (KIND == RB_Y) ? 1 : 0;
     const double alpha = vs->alpha, beta = vs->beta;
     const bool first = vs->first != 0;
-    const bool first_res = vs->refinements == 0;   // x64 not yet set: use x0
-    const bool keep = vs->refinements > 0 && !vs->discard;
-    const int nprim = d.primary == 0 ? d.nx : (d.primary == 1 ? d.ny : d.nz);
     const bool odd = (vs->phase & 1) != 0;
-    float* p_old = L.p[odd ? 1 : 0] + d.off_c;
+    const float* p_old = L.p[odd ? 1 : 0] + d.off_c;
     float* p_new = L.p[odd ? 0 : 1] + d.off_c;
     float* X = L.x + d.off_c;
     float* Rr = L.r + d.off_c;
     float* Q = L.q + d.off_c;
     double* Y = L.y + d.off_c;
+    const uint8_t* code_c = d.code_c[kColour];
+    const unsigned c00 = rb_index(d, 0, 0, 0);
+    auto valid = [&](int z, int& s, int& x) {
+        s = (kColour + ps.y + z) & 1;
+        x = 2 * ps.k + s;
+        return ps.row_ok && z < ps.z1 && x < d.nx;
+    };
+    auto load = [&](int z, RbIn& in) {
+        int s, x;
+        const unsigned ci = valid(z, s, x) ? rb_index(d, ps.k, ps.y, z) : c00;
+        in.code = ld(code_c, ci);
+        if (KIND == RB_P) {
+            in.a = ld(Rr, ci);
+            in.b = ld(p_old, ci);   // unused on the first iteration
+        } else if (KIND == RB_Y) {
+            rb_nbrs(p_new, d, ci, s, in.u);
+        } else if (KIND == RB_Q) {
+            in.a = ld(p_new, ci);
+            rb_nbrs(Y, d, ci, s, in.w);
+        } else {
+            in.a = ld(p_new, ci);
+            in.b = ld(X, ci);
+            in.c = ld(Rr, ci);
+            in.e = ld(Q, ci);
+        }
+    };
+    AccDD acc[2];
+    acc[0].init();
+    acc[1].init();
+    auto proc = [&](int z, const RbIn& in) {
+        int s, x;
+        if (!valid(z, s, x)) return;
+        const unsigned ci = rb_index(d, ps.k, ps.y, z);
+        const int code = in.code;
+        const int bx = box_flags(x, ps.y, z, d.nx, d.ny, d.nz);
+        const DiagInv<double> di = diag_at<double>(tabs.tab, code, bx, d.primary, c);
+        if (KIND == RB_P) {
+            const double z_ = di.iv * (double)in.a;
+            const double pn = first ? z_ : z_ + beta * (double)in.b;
+            st(p_new, ci, (float)pn);
+        } else if (KIND == RB_Y) {
+            st(Y, ci, rb_sum(tabs.ftab, code, in.u) * di.iv);
+        } else if (KIND == RB_Q) {
+            const double pv = (double)in.a;
+            const double qv = di.d * pv - rb_sum(tabs.ftab, code, in.w);
+            st(Q, ci, (float)qv);
+            acc[0].add(pv, qv);
+        } else {
+            const double pv = (double)in.a;
+            st(X, ci, (float)((double)in.b + alpha * pv));
+            const float rs = (float)((double)in.c - alpha * (double)in.e);
+            st(Rr, ci, rs);
+            acc[0].add(rs, rs);
+            acc[1].add((double)rs, di.iv * (double)rs);
+        }
+    };
+    RbIn i0, i1, i2;
+    load(ps.z0, i0);
+    load(ps.z0 + 1, i1);
+    // no early exit: a step past the last plane loads the first slot and does nothing, which
+    // keeps the loop body straight (the compiler then waits for each set's own loads only)
+    for (int z = ps.z0; z < ps.z1; z += 3) {
+        load(z + 2, i2);
+        proc(z, i0);
+        load(z + 3, i0);
+        proc(z + 1, i1);
+        load(z + 4, i1);
+        proc(z + 2, i2);
+    }
+    if (KIND == RB_Q) block_reduce_store<AccDD, 1, kRbTX * kRbTY>(reinterpret_cast<AccDD(&)[1]>(acc), L.part + (long long)tile * kPart);
+    if (KIND == RB_U) block_reduce_store<AccDD, 2, kRbTX * kRbTY>(acc, L.part + (long long)tile * kPart);
+}
+
+// Q with the black Y planes staged in LDS: a block owns kRbTX slots x kRbTY rows of red
+// voxels and marches z, keeping black planes z - 1, z, z + 1 over its tile plus a one-slot,
+// one-row halo in a four-plane ring.  Per plane a thread loads at most two Y values (the halo
+// plane in block-wide order) instead of six neighbours.  With four ring planes one barrier per
+// plane suffices: the plane written at step z + 1 is not among those read at step z, and the
+// one written at step z + 2 can only be written once every thread has passed step z + 1's
+// barrier.  Same values, roundings and summation order as k_rb_it<RB_Q>.  (Staging p for the
+// Y pass the same way measured slower than k_rb_it<RB_Y>: its loads are 4-byte.)
+__global__ __launch_bounds__(kRbTX* kRbTY) void k_rb_q(RbLaunch L) {
+    using U = double;
+    constexpr int W = kRbTX + 2, PL = W * (kRbTY + 2);
+    constexpr int NT = kRbTX * kRbTY;
+    constexpr int RND = (PL + NT - 1) / NT;
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
+    if (tile >= L.ntiles) return;
+    const int v = L.tile_vol[tile];
+    const RbDesc d = L.vol[v];
+    const VolState<double>* vs = L.st + v;
+    if (vs->status != ST_RUNNING) return;
+    __shared__ RbTabs tabs;
+    __shared__ U ring[4 * PL];
+    const Coef<double> c = L.coef[v];
+    rb_load_tables(L, v, c, tabs);
+    const int lt = tile - d.tile_base;
+    const int k0 = (lt % d.tiles_x) * kRbTX;
+    const int y0 = ((lt / d.tiles_x) % d.tiles_y) * kRbTY;
+    const int z0 = (lt / (d.tiles_x * d.tiles_y)) * d.tz;
+    const int z1 = min(z0 + d.tz, d.nz);
+    const bool odd = (vs->phase & 1) != 0;
+    const float* P = L.p[odd ? 0 : 1] + d.off_c;
+    const U* src = L.y + d.off_c;
+    const uint8_t* code_c = d.code_c[0];
+    // element of halo point j of this thread in black plane zc (-1 <= zc <= nz: the ghost layer
+    // covers the box's neighbours); points that no red voxel reads load element 0
+    unsigned hofs[RND];
+#pragma unroll
+    for (int j = 0; j < RND; ++j) {
+        const int i = (int)threadIdx.x + j * NT;
+        const int kk = k0 - 1 + i % W, yy = y0 - 1 + i / W;
+        hofs[j] = (i < PL && kk <= d.hx && yy <= d.ny) ? (unsigned)(kk + 1) + (unsigned)d.px * (unsigned)(yy + 1) : ~0u;
+    }
+    auto h_load = [&](int zc, U (&h)[RND]) {
+#pragma unroll
+        for (int j = 0; j < RND; ++j) {
+            const bool ok = hofs[j] != ~0u && zc <= d.nz;
+            h[j] = ld(src, ok ? hofs[j] + d.pp * (unsigned)(zc + 1) : 0u);
+        }
+    };
+    auto h_store = [&](int zc, const U (&h)[RND]) {
+        U* dst = ring + (zc & 3) * PL;
+#pragma unroll
+        for (int j = 0; j < RND; ++j) {
+            const int i = (int)threadIdx.x + j * NT;
+            if (i < PL) dst[i] = h[j];
+        }
+    };
+    const int k = k0 + (int)(threadIdx.x % kRbTX), y = y0 + (int)(threadIdx.x / kRbTX);
+    const bool row_ok = k < d.hx && y < d.ny;
+    const int col = (k - k0 + 1) + (y - y0 + 1) * W;   // this thread's slot in a ring plane
+    const unsigned c00 = rb_index(d, 0, 0, 0);
+    auto c_load = [&](int z, int& code, float& pv) {
+        const int x = 2 * k + ((y + z) & 1);
+        const unsigned ci = (row_ok && z < z1 && x < d.nx) ? rb_index(d, k, y, z) : c00;
+        code = ld(code_c, ci);
+        pv = ld(P, ci);
+    };
+    AccDD acc[1];
+    acc[0].init();
+    auto plane = [&](int z, int code, float pv) {
+        const int s = (y + z) & 1;
+        const int x = 2 * k + s;
+        if (!(row_ok && z < z1 && x < d.nx)) return;
+        const int bx = box_flags(x, y, z, d.nx, d.ny, d.nz);
+        const DiagInv<double> di = diag_at<double>(tabs.tab, code, bx, d.primary, c);
+        const U* pl = ring + (z & 3) * PL + col;
+        U o[6];
+        o[0] = ring[((z - 1) & 3) * PL + col];
+        o[1] = pl[-W];
+        o[2] = pl[s - 1];
+        o[3] = pl[s];
+        o[4] = pl[W];
+        o[5] = ring[((z + 1) & 3) * PL + col];
+        const double p64 = (double)pv;
+        const double qv = di.d * p64 - rb_sum(tabs.ftab, code, o);
+        st(L.q + d.off_c, rb_index(d, k, y, z), (float)qv);
+        acc[0].add(p64, qv);
+    };
+    U ha[RND], hb[RND];
+    h_load(z0 - 1, ha);
+    h_store(z0 - 1, ha);
+    h_load(z0, ha);
+    h_store(z0, ha);
+    h_load(z0 + 1, ha);
+    int code_a, code_b;
+    float pv_a, pv_b;
+    c_load(z0, code_a, pv_a);
+    // unrolled by 2 over alternating register sets, no early exit (a step past the last plane
+    // stores an unused ring plane and computes nothing)
+    for (int z = z0; z < z1; z += 2) {
+        h_store(z + 1, ha);
+        h_load(z + 2, hb);
+        c_load(z + 1, code_b, pv_b);
+        __syncthreads();
+        plane(z, code_a, pv_a);
+        h_store(z + 2, hb);
+        h_load(z + 3, ha);
+        c_load(z + 2, code_a, pv_a);
+        __syncthreads();
+        plane(z + 1, code_b, pv_b);
+    }
+    block_reduce_store<AccDD, 1, NT>(acc, L.part + (long long)tile * kPart);
+}
+
+// The passes outside the Krylov loop, one thread per slot of a row, marching tz planes:
+//   CODE:    per-colour stencil codes from the full-grid codes (set-up)
+//   UPD:     x64 += x (kept unless discarded), x = 0                      [update / fresh start]
+//   Y64:     black x64b = D_b^-1 (b_b - A_br x64), partial |b_b|^2       [update / fresh start]
+//   R64:     red r = b_S - S x64 = (b_r - D_r x64) - A_rb x64b in FP64, stored in FP32;
+//            partials |r64|^2, |b_r|^2, |r32|^2, r32.(D^-1 r32)           [update / fresh start]
+//   SCATTER: both colours of x64 / x64b into the full-grid solution        [end]
+// On the first update x64 is the reference's initial guess x0, which depends only on the
+// primary coordinate: Y64 evaluates it for the red neighbours, R64 stores it.
+template <int KIND>
+__global__ __launch_bounds__(kRbTX* kRbTY) void k_rb(RbLaunch L) {
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
+    if (tile >= L.ntiles) return;
+    const int v = L.tile_vol[tile];
+    const RbDesc d = L.vol[v];
+    VolState<double>* vs = L.st + v;
+    if (KIND == RB_UPD || KIND == RB_Y64 || KIND == RB_R64) {
+        const int status = vs->status;
+        if (status != ST_UPDATE && status != ST_FRESH) return;
+    }
+    constexpr bool kTables = (KIND == RB_Y64 || KIND == RB_R64);
+    __shared__ RbTabs tabs;
+    const Coef<double> c = L.coef[v];
+    if (kTables) rb_load_tables(L, v, c, tabs);
+    const RbPos ps = rb_pos(d, tile - d.tile_base);
+    constexpr int kColour = (KIND == RB_Y64) ? 1 : 0;
+    const bool first_res = vs->refinements == 0;   // x64 not yet set: use x0
+    const bool keep = vs->refinements > 0 && !vs->discard;
+    const int nprim = d.primary == 0 ? d.nx : (d.primary == 1 ? d.ny : d.nz);
+    float* X = L.x + d.off_c;
+    float* Rr = L.r + d.off_c;
     double* X64 = L.x64 + d.off_c;
     double* X64B = L.x64b + d.off_c;
     AccDD acc[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) acc[i].init();
     if (ps.row_ok) {
-        const unsigned plane_c = (unsigned)d.hx * (unsigned)d.ny;
         for (int z = ps.z0; z < ps.z1; ++z) {
-            const int s = ((kBlack ? 1 : 0) + ps.y + z) & 1;
+            const int s = (kColour + ps.y + z) & 1;
             const int x = 2 * ps.k + s;
-            const unsigned ci = (unsigned)ps.k + (unsigned)d.hx * (unsigned)ps.y + plane_c * (unsigned)z;
+            const unsigned ci = rb_index(d, ps.k, ps.y, z);
             if (KIND == RB_SCATTER) {   // slot k of both colours: red x = 2k + s, black x = 2k + 1 - s
                 double* row = L.xfull + d.off + (long long)d.nx * ((long long)ps.y + (long long)d.ny * z);
                 if (x < d.nx) row[x] = X64[ci];
                 if (2 * ps.k + 1 - s < d.nx) row[2 * ps.k + 1 - s] = X64B[ci];
                 continue;
             }
+            if (KIND == RB_CODE) {   // both colours
+                const unsigned lin = (unsigned)d.nx * ((unsigned)ps.y + (unsigned)d.ny * (unsigned)z);
+                uint8_t* cr = const_cast<uint8_t*>(d.code_c[0]);
+                uint8_t* cb = const_cast<uint8_t*>(d.code_c[1]);
+                if (x < d.nx) cr[ci] = d.code[lin + (unsigned)x];
+                if (2 * ps.k + 1 - s < d.nx) cb[ci] = d.code[lin + (unsigned)(2 * ps.k + 1 - s)];
+                continue;
+            }
             if (x >= d.nx) continue;
-            const unsigned lin = (unsigned)x + (unsigned)d.nx * ((unsigned)ps.y + (unsigned)d.ny * (unsigned)z);
             if (KIND == RB_UPD) {
                 if (keep) X64[ci] = X64[ci] + (double)X[ci];
                 X[ci] = 0.0f;
                 continue;
             }
-            const int code = ld(d.code, lin);
+            const int code = ld(d.code_c[kColour], ci);
             const int bx = box_flags(x, ps.y, z, d.nx, d.ny, d.nz);
             const DiagInv<double> di = diag_at<double>(tabs.tab, code, bx, d.primary, c);
             const int kp = d.primary == 0 ? x : (d.primary == 1 ? ps.y : z);
-            if (KIND == RB_P) {
-                const double z_ = di.iv * (double)ld(Rr, ci);
-                const double pn = first ? z_ : z_ + beta * (double)ld(p_old, ci);
-                st(p_new, ci, (float)pn);
-            } else if (KIND == RB_Y) {
-                const double g = rb_gather(p_new, d, tabs.ftab, code, ps.k, s, x, ps.y, z);
-                st(Y, ci, g / di.d);
-            } else if (KIND == RB_Q) {
-                const double pv = (double)ld(p_new, ci);
-                const double g = rb_gather(Y, d, tabs.ftab, code, ps.k, s, x, ps.y, z);
-                const double qv = di.d * pv - g;
-                st(Q, ci, (float)qv);
-                acc[0].add(pv, qv);
-            } else if (KIND == RB_U) {
-                const double pv = (double)ld(p_new, ci);
-                st(X, ci, (float)((double)ld(X, ci) + alpha * pv));
-                const float rs = (float)((double)ld(Rr, ci) - alpha * (double)ld(Q, ci));
-                st(Rr, ci, rs);
-                acc[0].add(rs, rs);
-                acc[1].add((double)rs, di.iv * (double)rs);
-            } else if (KIND == RB_Y64) {
-                // black: x_b = (b_b - sum c x_r) / D_b from the FP64 red solution (x0 before the first update)
+            if (KIND == RB_Y64) {
                 double g;
-                if (KIND == RB_Y64 && first_res) {
-                    // red neighbours at x0: x0 depends only on the primary coordinate
+                if (first_res) {
                     const int o4 = (code >> 4) & 4;
                     const FacePair<double> px = tabs.ftab[(code & 3) | o4];
                     const FacePair<double> py = tabs.ftab[((code >> 2) & 3) | o4];
@@ -1754,7 +1970,9 @@ __global__ __launch_bounds__(kRbTX* kRbTY) void k_rb(RbLaunch L) {
                     g += py.p * x0n(0, 1, 0, ps.y + 1 < d.ny);
                     g += pz.p * x0n(0, 0, 1, z + 1 < d.nz);
                 } else {
-                    g = rb_gather(X64, d, tabs.ftab, code, ps.k, s, x, ps.y, z);
+                    double o[6];
+                    rb_nbrs(X64, d, ci, s, o);
+                    g = rb_sum(tabs.ftab, code, o);
                 }
                 const double b = rhs_of<double>(code, kp, nprim, c);
                 st(X64B, ci, (b - g) / di.d);
@@ -1763,8 +1981,9 @@ __global__ __launch_bounds__(kRbTX* kRbTY) void k_rb(RbLaunch L) {
                 const double xr = first_res ? x0_of<double>(kp, nprim, d.sgn) : ld(X64, ci);
                 if (first_res) st(X64, ci, xr);
                 const double b = rhs_of<double>(code, kp, nprim, c);
-                const double g = rb_gather(X64B, d, tabs.ftab, code, ps.k, s, x, ps.y, z);
-                const double r64 = (b - di.d * xr) - g;
+                double o[6];
+                rb_nbrs(X64B, d, ci, s, o);
+                const double r64 = (b - di.d * xr) - rb_sum(tabs.ftab, code, o);
                 const float r32 = (float)r64;
                 st(Rr, ci, r32);
                 acc[0].add(r64, r64);
@@ -1774,8 +1993,6 @@ __global__ __launch_bounds__(kRbTX* kRbTY) void k_rb(RbLaunch L) {
             }
         }
     }
-    if (KIND == RB_Q) block_reduce_store<AccDD, 1, kRbTX * kRbTY>(reinterpret_cast<AccDD(&)[1]>(acc), L.part + (long long)tile * kPart);
-    if (KIND == RB_U) block_reduce_store<AccDD, 2, kRbTX * kRbTY>(reinterpret_cast<AccDD(&)[2]>(acc), L.part + (long long)tile * kPart);
     if (KIND == RB_Y64) block_reduce_store<AccDD, 1, kRbTX * kRbTY>(reinterpret_cast<AccDD(&)[1]>(acc), L.part_b + (long long)tile * kPart);
     if (KIND == RB_R64) block_reduce_store<AccDD, 4, kRbTX * kRbTY>(acc, L.part + (long long)tile * kPart);
 }
@@ -1828,20 +2045,33 @@ void rb_launch(const RbLaunch& L, hipStream_t s) {
     hipLaunchKernelGGL((k_rb<KIND>), dim3(L.ntiles), dim3(kRbTX * kRbTY), 0, s, L);
 }
 
+template <int KIND>
+void rb_launch_it(const RbLaunch& L, hipStream_t s) {
+    hipLaunchKernelGGL((k_rb_it<KIND>), dim3(L.ntiles), dim3(kRbTX * kRbTY), 0, s, L);
+}
+
 }  // namespace
 
-hipError_t rb_tables(const RbLaunch& L, hipStream_t s) {
+hipError_t rb_setup(const RbLaunch& L, hipStream_t s) {
     if (L.nvol <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_rb_tables, dim3(L.nvol), dim3(256), 0, s, L.vol, L.coef, L.tab, L.nvol);
+    rb_launch<RB_CODE>(L, s);
     return hipGetLastError();
 }
 
+static void rb_launch_q(const RbLaunch& L, hipStream_t s) {
+    if (L.staged)
+        hipLaunchKernelGGL(k_rb_q, dim3(L.ntiles), dim3(kRbTX * kRbTY), 0, s, L);
+    else
+        rb_launch_it<RB_Q>(L, s);
+}
+
 hipError_t rb_iter(const RbLaunch& L, hipStream_t s) {
-    rb_launch<RB_P>(L, s);
-    rb_launch<RB_Y>(L, s);
-    rb_launch<RB_Q>(L, s);
+    rb_launch_it<RB_P>(L, s);
+    rb_launch_it<RB_Y>(L, s);
+    rb_launch_q(L, s);
     hipLaunchKernelGGL((k_rb_fin<F_CG1>), dim3(L.nvol), dim3(kFinThreads), 0, s, L);
-    rb_launch<RB_U>(L, s);
+    rb_launch_it<RB_U>(L, s);
     hipLaunchKernelGGL((k_rb_fin<F_CG2>), dim3(L.nvol), dim3(kFinThreads), 0, s, L);
     return hipGetLastError();
 }
@@ -1851,6 +2081,17 @@ hipError_t rb_update(const RbLaunch& L, hipStream_t s) {
     rb_launch<RB_Y64>(L, s);
     rb_launch<RB_R64>(L, s);
     hipLaunchKernelGGL((k_rb_fin<F_MIXRES>), dim3(L.nvol), dim3(kFinThreads), 0, s, L);
+    return hipGetLastError();
+}
+
+hipError_t rb_pass_only(const RbLaunch& L, int k, hipStream_t s) {
+    switch (k) {
+        case 0: rb_launch_it<RB_P>(L, s); break;
+        case 1: rb_launch_it<RB_Y>(L, s); break;
+        case 2: rb_launch_q(L, s); break;
+        case 3: rb_launch_it<RB_U>(L, s); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

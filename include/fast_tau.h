@@ -221,6 +221,11 @@ int fast_debug_system_f64(fast_volume* vol, int dir, double* b, double* x0, doub
  * period of a whole iteration (passes and per-volume finalizes) over `iters` iterations.
  * ms: 4 doubles; BiCGSTAB {pass1, pass2, pass3, iteration}, CG {pass1, pass2, 0, iteration}. */
 int fast_debug_time_passes(fast_volume* vol, int dir, int precision, int solver, int iters, double* ms);
+/* The same for mixed CG on the red-black Schur complement: ms[0..3] = mean duration of the
+ * passes P (p = D^-1 r + beta p), Y (black: D^-1 A p), Q (red: q = S p, p.q) and U (x, r
+ * update, r.r, r.D^-1 r), each launched `iters` times back to back; ms[4] = mean period of a
+ * whole iteration.  ms: 5 doubles. */
+int fast_debug_time_rb_passes(fast_volume* vol, int dir, int iters, double* ms);
 
 #ifdef __cplusplus
 }

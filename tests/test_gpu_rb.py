@@ -116,3 +116,25 @@ def test_reduced_batch_equals_solo_uneven(ctx):
         assert b.status == 0 and solo.status == 0
         assert b.tau == solo.tau and b.iterations == solo.iterations and b.refinements == solo.refinements
         v.close()
+
+
+def test_staged_q_pass_bit_identical(ctx, monkeypatch):
+    """The Q pass with the black Y planes staged in LDS and the one that reads the six Y
+    neighbours from memory compute the same values in the same order: tau, iterations and the
+    concentration field are identical, on a cube and on odd shapes whose tiles are ragged in
+    every axis."""
+    cases = [F.generate_spheres(64, 6.0, 0.60, 0), rand_mask((19, 23, 37), 0.45, 1), rand_mask((11, 17, 131), 0.4, 5),
+             rand_mask((70, 5, 1), 0.3, 3)]
+    for m in cases:
+        vol = ctx.volume(m)
+        for d in (0, 3, 4):
+            p = T.make_params(d, CG, "mixed", 1e-9, 10000)
+            monkeypatch.setenv("FAST_TAU_RB_STAGED", "1")
+            a, ca = vol.solve(p, conc=True)
+            monkeypatch.setenv("FAST_TAU_RB_STAGED", "0")
+            b, cb = vol.solve(p, conc=True)
+            monkeypatch.delenv("FAST_TAU_RB_STAGED")
+            assert a.status == 0 and b.status == 0
+            assert a.tau == b.tau and a.iterations == b.iterations, (m.shape, d, a.tau, b.tau)
+            assert np.array_equal(ca, cb), (m.shape, d)
+        vol.close()
