@@ -5,9 +5,11 @@ semi-axes (12, 4.2, 4.2) (a = 12, r = 0.35), uniformly random orientations and c
 filled to solid fraction 0.65 (seed 0); tau in the x, y and z directions at tolerance 1e-6.
 
   * plain FP64 GPU solve (the drop-in): BiCGSTAB and CG, one direction per solve;
-  * mixed + batched (the headline form): CG with FP32 vectors in HBM, FP64 arithmetic and
-    dot products, the solution accumulated in FP64 with reliable updates of the residual;
-    the three directions solved as one batched solve.
+  * mixed + batched (the headline form): Jacobi-preconditioned CG on the red-black Schur
+    complement of the same linear system, FP32 vectors in HBM, FP64 arithmetic and dot
+    products, the solution accumulated in FP64 with reliable updates against the FP64 true
+    residual of the full system; the three directions solved as one batched solve;
+  * the same mixed CG on the full grid (for comparison).
 A "step" is one batched mixed solve of the three directions; value = mixed-batched solves/s.
 Inputs are resident in HBM (fast_volume) before the timed region.
 
@@ -126,14 +128,15 @@ def serving(ctx, T, W, n_req: int, loads=(0.5, 0.8, 1.1)) -> dict:
     return out
 
 
-def measured_traffic(applies: bool):
-    """HBM bytes per launch of the mixed PCG pass-2 kernel at 256^3, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE counters (copy-calibrated); None when not measured."""
-    path = ROOT / "profiles" / "cg2_mixed_256_traffic.txt"
+def measured_traffic(kernel: str, applies: bool):
+    """HBM bytes per launch of `kernel` at 256^3, from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE counters (copy-calibrated); None when not measured."""
+    path = ROOT / "profiles" / "rb_passes_256_traffic.txt"
     if not applies or not path.exists():
         return None
     kv = dict(line.split("=", 1) for line in path.read_text().splitlines() if "=" in line and not line.startswith("#"))
-    return float(kv["traffic_bytes"])
+    key = f"traffic_bytes[{kernel}]"
+    return float(kv[key]) if key in kv else None
 
 
 def profile_program(kind: str, n: int) -> int:
@@ -146,9 +149,10 @@ def profile_program(kind: str, n: int) -> int:
     ctx = T.Context(0)
     vol = ctx.volume(mask)
     if kind == "solves":
-        for name, solver, prec in (("fp64 bicgstab", BICG, "double"), ("fp64 cg", CG, "double"),
-                                   ("mixed cg", CG, "mixed")):
-            res, _ = vol.solve(T.make_params(0, solver, prec, TOL, 10000))
+        for name, solver, prec, full in (("fp64 bicgstab", BICG, "double", False), ("fp64 cg", CG, "double", False),
+                                         ("mixed cg red-black", CG, "mixed", False),
+                                         ("mixed cg full grid", CG, "mixed", True)):
+            res, _ = vol.solve(T.make_params(0, solver, prec, TOL, 10000, full_grid=full))
             print(f"{name}: tau {res.tau!r} iterations {res.iterations} {res.solve_ms:.1f} ms", flush=True)
     else:
         ms = (C.c_double * 4)()
@@ -180,9 +184,10 @@ def main() -> int:
     ap.add_argument("--cpu-iters", type=int, default=25)
     ap.add_argument("--serve", type=int, default=120, help="requests per offered load (0: skip serving)")
     ap.add_argument("--profile", choices=["solves", "passes"],
-                    help="short program for rocprofv3 instead of the benchmark: 'solves' runs the three X_POS "
-                         "solves (FP64 BiCGSTAB, FP64 CG, mixed CG) with direct launches; 'passes' times 20 "
-                         "iterations of each and runs a 256 MiB device copy for counter calibration")
+                    help="short program for rocprofv3 instead of the benchmark: 'solves' runs the X_POS solves "
+                         "(FP64 BiCGSTAB, FP64 CG, mixed CG red-black and full grid) with direct launches; 'passes' "
+                         "times 20 launches of each solver's passes and runs a 256 MiB device copy for counter "
+                         "calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -233,27 +238,42 @@ def main() -> int:
     value = len(DIRS) * args.steps / elapsed
     err_mixed = {d: abs(o.tau - conv[d]) / conv[d] for o, d in zip(out, DIRS)}
 
+    # ---- the same batch with mixed CG on the full grid ----
+    params_full = [T.make_params(d, CG, "mixed", TOL, 10000, full_grid=True) for d in DIRS]
+    T.solve_batch(ctx, vols, params_full)
+    t0 = time.perf_counter()
+    out_full = T.solve_batch(ctx, vols, params_full)
+    full_rate = len(DIRS) / (time.perf_counter() - t0)
+    err_full = {d: abs(o.tau - conv[d]) / conv[d] for o, d in zip(out_full, DIRS)}
+
     # ---- roofline of the dominant kernel, HIP events on the solve stream ----
     ms = (C.c_double * 4)()
     lib().fast_debug_time_passes(vol.handle, 0, 2, CG, 20, ms)
-    mixed_pass = list(ms)
+    mixed_full_pass = list(ms)
     lib().fast_debug_time_passes(vol.handle, 0, 0, CG, 20, ms)
     fp64_cg_pass = list(ms)
     lib().fast_debug_time_passes(vol.handle, 0, 0, BICG, 20, ms)
     fp64_bicg_pass = list(ms)
-    # mixed CG: pass 1 reads r, p_old and the code byte and writes p (3 FP32 vectors);
-    # pass 2 reads p (stencil), x, r and the code byte and writes x, r (5 FP32 vectors)
+    rb = (C.c_double * 5)()
+    lib().fast_debug_time_rb_passes(vol.handle, 0, 20, rb)
+    rb_pass = list(rb)
+    # algorithmic bytes per launch (each vector element and code byte once): FP32 r, p, x, q
+    # and FP64 Y, 1-byte codes, per red (n_r) and black (n_b) voxel
+    n_r = (nvox + 1) // 2 if all(e % 2 for e in mask.shape) else nvox // 2
+    n_b = nvox - n_r
     kernels = {
-        "k_stencil<float,double,64,8,K_CG1> (PCG pass 1: p = M^-1 r + beta p, p.Ap)":
-            (mixed_pass[0], (3 * 4 + 1) * nvox),
-        "k_stencil<float,double,64,8,K_CG2> (PCG pass 2: q = Ap, x += alpha p, r -= alpha q, r.r, r.M^-1 r)":
-            (mixed_pass[1], (5 * 4 + 1) * nvox),
+        "k_rb_it<0> (P, red: p = D^-1 r + beta p)": (rb_pass[0], (4 + 4 + 1 + 4) * n_r),
+        "k_rb_it<1> (Y, black: Y = D_b^-1 A_br p)": (rb_pass[1], (1 + 8) * n_b + 4 * n_r),
+        "k_rb_q (Q, red: q = S p = D_r p - A_rb Y, p.q)": (rb_pass[2], (1 + 4 + 4) * n_r + 8 * n_b),
+        "k_rb_it<3> (U, red: x += alpha p, r -= alpha q, r.r, r.D^-1 r)": (rb_pass[3], (4 * 4 + 1 + 2 * 4) * n_r),
     }
     name, (kms, kbytes) = max(kernels.items(), key=lambda kv: kv[1][0])
     ach = kbytes / (kms * 1e-3) / 1e9
-    traffic = measured_traffic("K_CG2" in name and n == 256)
-    others = {k: {"avg_launch_ms": v[0], "bytes_per_launch": v[1], "achieved_GBs": v[1] / (v[0] * 1e-3) / 1e9}
+    traffic = measured_traffic(name.split(" ")[0], n == 256)
+    others = {k: {"avg_launch_ms": v[0], "bytes_per_launch": v[1], "achieved_GBs": v[1] / (v[0] * 1e-3) / 1e9,
+                  "traffic": measured_traffic(k.split(" ")[0], n == 256)}
               for k, v in kernels.items()}
+    iter_bytes = sum(v[1] for v in kernels.values())
 
     cpu = None
     if not args.no_cpu:
@@ -282,10 +302,11 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": f"{n}^3 ellipsoids a=12 r=0.35 solid 0.65 seed 0; tol 1e-6; x,y,z solved as one "
-                        "batched mixed-precision CG solve per step",
+                        "batched mixed-precision CG solve (red-black Schur complement) per step",
             "volume": [n, n, n], "solves_per_step": len(DIRS), "parallelism": "single GPU",
         },
         "mixed_batched_ms_per_solve": ms_per_step / len(DIRS),
+        "mixed_full_grid_batched_solves_per_s": full_rate,
         "fp64_bicgstab_solves_per_s": fp64_rate[BICG],
         "fp64_cg_solves_per_s": fp64_rate[CG],
         "speedup_vs_fastest_fp64": value / best_fp64,
@@ -294,11 +315,13 @@ def main() -> int:
         "converged_tau_source": conv_src,
         "tau_rel_err_vs_converged": {
             "mixed_batched_cg": {DIRNAMES[d]: err_mixed[d] for d in DIRS},
+            "mixed_full_grid_cg": {DIRNAMES[d]: err_full[d] for d in DIRS},
             "fp64_bicgstab": {DIRNAMES[d]: abs(fp64[(BICG, d)][0].tau - conv[d]) / conv[d] for d in DIRS},
             "fp64_cg": {DIRNAMES[d]: abs(fp64[(CG, d)][0].tau - conv[d]) / conv[d] for d in DIRS},
         },
         "iterations": {
             "mixed_batched_cg": {DIRNAMES[d]: int(o.iterations) for o, d in zip(out, DIRS)},
+            "mixed_full_grid_cg": {DIRNAMES[d]: int(o.iterations) for o, d in zip(out_full, DIRS)},
             "fp64_bicgstab": {DIRNAMES[d]: int(fp64[(BICG, d)][0].iterations) for d in DIRS},
             "fp64_cg": {DIRNAMES[d]: int(fp64[(CG, d)][0].iterations) for d in DIRS},
         },
@@ -308,7 +331,10 @@ def main() -> int:
             "timing": "HIP events on the solve stream around 20 back-to-back launches of the kernel",
         },
         "kernels_mixed_cg": others,
-        "pass_ms": {"mixed_cg": mixed_pass, "fp64_cg": fp64_cg_pass, "fp64_bicgstab": fp64_bicg_pass},
+        "mixed_cg_iteration": {"ms": rb_pass[4], "algorithmic_bytes": iter_bytes,
+                               "achieved_GBs": iter_bytes / (rb_pass[4] * 1e-3) / 1e9},
+        "pass_ms": {"mixed_cg_red_black": rb_pass, "mixed_cg_full_grid": mixed_full_pass, "fp64_cg": fp64_cg_pass,
+                    "fp64_bicgstab": fp64_bicg_pass},
         "cpu_baseline": cpu,
         "serving": serve,
     }
