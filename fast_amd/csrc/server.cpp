@@ -6,11 +6,12 @@
 // admits queued requests while a slot and room in the device arena remain (admission by HBM
 // budget), uploads their masks and builds their stencil codes, and adds their tiles to the
 // launch.  Every round runs `iters_per_round` iterations of the mixed-form
-// kernels over all live volumes at once; each volume carries its own scalars and status, so
-// volumes join and leave at round boundaries without disturbing the others.  Finished volumes
-// get their flux integral and tau, release their memory, and are handed to fast_server_wait.
-// All dot products are double-double (tile- and batch-invariant), so a request's result is
-// bit-identical to solving it alone.
+// kernels over all live volumes at once (CG on the red-black Schur complement, or BiCGSTAB on
+// the full grid); each volume carries its own scalars and status, so volumes join and leave at
+// round boundaries without disturbing the others.  Finished volumes get their flux integral
+// and tau, release their memory, and are handed to fast_server_wait.  All dot products are
+// double-double (tile- and batch-invariant) and each volume's tiling depends on its own
+// dimensions only, so a request's result is bit-identical to solving it alone.
 #include "fast_tau.h"
 #include "tau_host.h"
 #include "tau_internal.h"
@@ -35,9 +36,11 @@ double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-constexpr int kTX = 64, kTY = 8;      // one launch shape for the whole live set
+constexpr int kTX = 64, kTY = 8;      // one launch shape for the whole live set (full grid)
 constexpr size_t kAlign = 64;         // arena granularity (elements)
-constexpr size_t kPlanes = 8;         // FP32 vectors per request: x (the correction), r, r0, p[2], v[2], s
+constexpr size_t kPlanes = 8;         // full grid: FP32 vectors x (the correction), r, r0, p[2], v[2], s
+// red-black: per padded per-colour element, FP32 x, r, p[2], q, FP64 y, x64, x64b and two code bytes
+constexpr size_t kRbBytes = 5 * sizeof(float) + 3 * sizeof(double) + 2;
 
 struct Req {
     uint64_t ticket = 0;
@@ -49,7 +52,7 @@ struct Req {
     uint8_t* d_mask = nullptr;
     uint8_t* d_code = nullptr;
     int slot = -1;
-    size_t off = 0, elems = 0;
+    size_t off = 0, elems = 0;   // arena range (voxels, or padded per-colour elements for red-black)
     int tz = 16;
     fast_tau_result res{};
     int rc = FAST_OK;
@@ -101,6 +104,7 @@ struct fast_server {
     fast_ctx* ctx = nullptr;
     fast_server_config cfg{};
     int solver = FAST_CG;
+    bool rb = true;                           // CG on the red-black Schur complement
     std::thread worker;
     std::mutex mu;
     std::condition_variable cv_in, cv_out;
@@ -132,6 +136,16 @@ struct fast_server {
     std::vector<VolDesc> h_vol;
     VolState<double>* h_st = nullptr;         // pinned snapshot
     Launch<float, double> L{};
+    // red-black form
+    RbDesc* d_rb = nullptr;
+    std::vector<RbDesc> h_rb;
+    double* d_partb = nullptr;
+    int* d_tile_fin = nullptr;                // tiles of the finishing slots (scatter)
+    size_t tile_fin_cap = 0;
+    double* d_xfull = nullptr;                // full-grid solutions of the finishing slots
+    size_t xfull_cap = 0;
+    RbLaunch RL{};
+    int ntiles = 0;
 };
 
 namespace {
@@ -145,6 +159,10 @@ bool grow(fast_server* s, void** p, size_t* cap, size_t need, size_t elem) {
     return *p != nullptr;
 }
 
+float* rb_f32(fast_server* s, int k) { return (float*)s->arena_mem + (size_t)k * s->S; }
+double* rb_f64(fast_server* s, int k) { return (double*)((float*)s->arena_mem + 5 * s->S) + (size_t)k * s->S; }
+uint8_t* rb_u8(fast_server* s, int k) { return (uint8_t*)(rb_f64(s, 0) + 3 * s->S) + (size_t)k * s->S; }
+
 // Upload masks and stencil codes of newly admitted requests on the upload stream.
 int upload(fast_server* s, const std::vector<Req*>& adm) {
     fast_ctx* ctx = s->ctx;
@@ -156,6 +174,12 @@ int upload(fast_server* s, const std::vector<Req*>& adm) {
         hipError_t e = hipMemcpyAsync(r->d_mask, r->mask.data(), n, hipMemcpyHostToDevice, s->up);
         if (e == hipSuccess) e = hipMemsetAsync(s->d_zero + r->slot, 0, sizeof(unsigned long long), s->up);
         if (e == hipSuccess) e = ft::build_code(r->d_mask, r->d_code, r->nx, r->ny, r->nz, s->d_zero + r->slot, s->up);
+        // red-black: ghost and padding elements of the request's range must read +0
+        for (int k = 0; s->rb && k < 5 && e == hipSuccess; ++k)
+            e = hipMemsetAsync(rb_f32(s, k) + r->off, 0, r->elems * sizeof(float), s->up);
+        for (int k = 0; s->rb && k < 3 && e == hipSuccess; ++k)
+            e = hipMemsetAsync(rb_f64(s, k) + r->off, 0, r->elems * sizeof(double), s->up);
+        for (int k = 0; s->rb && k < 2 && e == hipSuccess; ++k) e = hipMemsetAsync(rb_u8(s, k) + r->off, 0, r->elems, s->up);
         if (!hip_ok(ctx, e, "server upload")) return FAST_E_HIP;
     }
     hipError_t e = hipEventRecord(s->up_done, s->up);
@@ -168,8 +192,108 @@ int upload(fast_server* s, const std::vector<Req*>& adm) {
     return FAST_OK;
 }
 
+// Initial state of a newly admitted slot (every request starts with the FP64 residual step).
+VolState<double> fresh_state(const fast_server* s, const Req* r) {
+    VolState<double> v{};
+    v.status = ST_FRESH;
+    v.mode = 1;
+    v.solver = s->solver;
+    v.tol = r->prm.tolerance;
+    v.maxiter = r->prm.max_iter;
+    v.otol = r->prm.mixed_outer_tol > 0 ? r->prm.mixed_outer_tol : r->prm.tolerance;
+    const double delta = (r->prm.mixed_inner_tol > 0 && r->prm.mixed_inner_tol < 1) ? r->prm.mixed_inner_tol : 0.1;
+    v.delta2 = delta * delta;
+    v.restart_cos = s->ctx->mixed_restart_cos;
+    v.eps2 = s->rb ? 2.220446049250313e-16 * 2.220446049250313e-16 : 1.1920928955078125e-07 * 1.1920928955078125e-07;
+    return v;
+}
+
+// Red-black form: descriptors and tile list of the live set (tile_base as the full launch
+// sees it), the per-slot states and coefficients of newly admitted requests, tables and codes.
+int rebuild_rb(fast_server* s, const std::vector<Req*>& adm) {
+    fast_ctx* ctx = s->ctx;
+    int tiles = 0;
+    std::vector<int> tv;
+    for (int i = 0; i < s->max_active; ++i) {
+        RbDesc& d = s->h_rb[i];
+        Req* r = s->slots[i];
+        d = RbDesc{};
+        d.tile_base = tiles;
+        if (!r) continue;
+        d.nx = r->nx;
+        d.ny = r->ny;
+        d.nz = r->nz;
+        d.hx = (r->nx + 1) / 2;
+        d.px = d.hx + 2;
+        d.pp = (unsigned)d.px * (unsigned)(r->ny + 2);
+        d.dir = r->prm.dir;
+        d.primary = r->prm.dir / 2;
+        d.sgn = (r->prm.dir % 2 == 0) ? 1 : -1;
+        d.tiles_x = (d.hx + kRbTX - 1) / kRbTX;
+        d.tiles_y = (r->ny + kRbTY - 1) / kRbTY;
+        d.tz = r->tz;
+        d.tiles_z = (r->nz + d.tz - 1) / d.tz;
+        d.ntiles = d.tiles_x * d.tiles_y * d.tiles_z;
+        d.off_c = (long long)r->off;
+        d.code = r->d_code;
+        d.code_c[0] = rb_u8(s, 0) + r->off;
+        d.code_c[1] = rb_u8(s, 1) + r->off;
+        for (int t = 0; t < d.ntiles; ++t) tv.push_back(i);
+        tiles += d.ntiles;
+    }
+    if (!grow(s, (void**)&s->d_tile, &s->tile_cap, std::max(tiles, 1), sizeof(int))) return FAST_E_OOM;
+    dfree(ctx, s->d_part);   // partials: 8 doubles per tile of the tile capacity, red and black
+    dfree(ctx, s->d_partb);
+    s->d_part = (double*)dalloc(ctx, s->tile_cap * 8 * sizeof(double));
+    s->d_partb = (double*)dalloc(ctx, s->tile_cap * 8 * sizeof(double));
+    if (!s->d_part || !s->d_partb) return FAST_E_OOM;
+    hipStream_t st = ctx->stream;
+    hipError_t e = hipStreamWaitEvent(st, s->up_done, 0);
+    if (e == hipSuccess && tiles)
+        e = hipMemcpyAsync(s->d_tile, tv.data(), sizeof(int) * tiles, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(s->d_rb, s->h_rb.data(), sizeof(RbDesc) * s->max_active, hipMemcpyHostToDevice, st);
+    for (Req* r : adm) {
+        if (e != hipSuccess) break;
+        const Coef<double> c = make_coef<double>(r->nx, r->ny, r->nz, r->prm.dir, r->prm.d_pore, r->prm.d_solid);
+        const VolState<double> v = fresh_state(s, r);
+        e = hipMemcpyAsync(s->d_coef + r->slot, &c, sizeof(c), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(s->d_st + r->slot, &v, sizeof(v), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);   // c and v live on this stack frame
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);       // tv and h_rb are read by the copies
+    if (!hip_ok(ctx, e, "server rebuild")) return FAST_E_HIP;
+    RbLaunch& L = s->RL;
+    L.ntiles = tiles;
+    L.nvol = s->max_active;
+    L.staged = 1;
+    L.vol = s->d_rb;
+    L.tile_vol = s->d_tile;
+    L.coef = s->d_coef;
+    L.st = s->d_st;
+    L.part = s->d_part;
+    L.part_b = s->d_partb;
+    L.tab = s->d_tab;
+    L.x = rb_f32(s, 0);
+    L.r = rb_f32(s, 1);
+    L.p[0] = rb_f32(s, 2);
+    L.p[1] = rb_f32(s, 3);
+    L.q = rb_f32(s, 4);
+    L.y = rb_f64(s, 0);
+    L.x64 = rb_f64(s, 1);
+    L.x64b = rb_f64(s, 2);
+    L.xfull = s->d_xfull;
+    s->ntiles = tiles;
+    if (!adm.empty()) {
+        const hipError_t et = ft::rb_setup(L, ctx->stream);   // tables of every slot, codes of the live set
+        if (!hip_ok(ctx, et, "server tables")) return FAST_E_HIP;
+    }
+    return FAST_OK;
+}
+
 // Descriptors and tile list of the live set; states of newly admitted slots.
 int rebuild(fast_server* s, const std::vector<Req*>& adm) {
+    if (s->rb) return rebuild_rb(s, adm);
     fast_ctx* ctx = s->ctx;
     int tiles = 0;
     std::vector<int> tv;
@@ -212,22 +336,14 @@ int rebuild(fast_server* s, const std::vector<Req*>& adm) {
     for (Req* r : adm) {
         if (e != hipSuccess) break;
         const Coef<double> c = make_coef<double>(r->nx, r->ny, r->nz, r->prm.dir, r->prm.d_pore, r->prm.d_solid);
-        VolState<double> v{};
-        v.status = ST_FRESH;
-        v.mode = 1;
-        v.solver = s->solver;
-        v.tol = r->prm.tolerance;
-        v.maxiter = r->prm.max_iter;
-        v.otol = r->prm.mixed_outer_tol > 0 ? r->prm.mixed_outer_tol : r->prm.tolerance;
-        const double delta = (r->prm.mixed_inner_tol > 0 && r->prm.mixed_inner_tol < 1) ? r->prm.mixed_inner_tol : 0.1;
-        v.delta2 = delta * delta;
-        v.restart_cos = ctx->mixed_restart_cos;
-        v.eps2 = 1.1920928955078125e-07 * 1.1920928955078125e-07;
+        const VolState<double> v = fresh_state(s, r);
         e = hipMemcpyAsync(s->d_coef + r->slot, &c, sizeof(c), hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = hipMemcpyAsync(s->d_st + r->slot, &v, sizeof(v), hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);   // c and v live on this stack frame
     }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);   // tv and h_vol are read by the copies
     if (!hip_ok(ctx, e, "server rebuild")) return FAST_E_HIP;
+    s->ntiles = tiles;
     Launch<float, double>& L = s->L;
     L.tx = kTX;
     L.ty = kTY;
@@ -260,6 +376,30 @@ int rebuild(fast_server* s, const std::vector<Req*>& adm) {
 // Flux and tau of the finished slots; release them.
 int retire(fast_server* s, const std::vector<int>& fin) {
     fast_ctx* ctx = s->ctx;
+    // red-black: both colours of each finishing slot go into a full-grid vector first; the
+    // scatter runs over the finishing slots' tiles only (tile_base and the full-grid offset
+    // as that launch sees them; the next rebuild restores the descriptors)
+    std::vector<int> ft_tiles;
+    size_t xn = 0;
+    if (s->rb) {
+        for (int i : fin) {
+            RbDesc& d = s->h_rb[i];
+            VolDesc& v = s->h_vol[i];
+            v = VolDesc{};
+            v.nx = d.nx;
+            v.ny = d.ny;
+            v.nz = d.nz;
+            v.dir = d.dir;
+            v.primary = d.primary;
+            v.sgn = d.sgn;
+            v.off = (long long)xn;
+            v.code = s->slots[i]->d_code;
+            d.off = (long long)xn;
+            d.tile_base = (int)ft_tiles.size();
+            for (int t = 0; t < d.ntiles; ++t) ft_tiles.push_back(i);
+            xn += align_up((size_t)d.nx * d.ny * d.nz, kAlign);
+        }
+    }
     std::vector<int> pv, pk;
     for (int i : fin) {
         const VolDesc& d = s->h_vol[i];
@@ -285,10 +425,28 @@ int retire(fast_server* s, const std::vector<int>& fin) {
     std::vector<double> planes(np_all);
     std::vector<unsigned long long> zeros(s->max_active);
     hipStream_t st = ctx->stream;
-    hipError_t e = hipMemcpyAsync(s->d_pvol, pv.data(), np_all * sizeof(int), hipMemcpyHostToDevice, st);
+    hipError_t e = hipSuccess;
+    const double* conc = s->L.vs.x64;
+    if (s->rb) {
+        if (!grow(s, (void**)&s->d_xfull, &s->xfull_cap, std::max<size_t>(xn, 1), sizeof(double)) ||
+            !grow(s, (void**)&s->d_tile_fin, &s->tile_fin_cap, std::max<size_t>(ft_tiles.size(), 1), sizeof(int)))
+            return FAST_E_OOM;
+        RbLaunch F = s->RL;
+        F.ntiles = (int)ft_tiles.size();
+        F.tile_vol = s->d_tile_fin;
+        F.xfull = s->d_xfull;
+        e = hipMemcpyAsync(s->d_rb, s->h_rb.data(), sizeof(RbDesc) * s->max_active, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess && F.ntiles)
+            e = hipMemcpyAsync(s->d_tile_fin, ft_tiles.data(), sizeof(int) * ft_tiles.size(), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(s->d_vol, s->h_vol.data(), sizeof(VolDesc) * s->max_active, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess && F.ntiles) e = ft::rb_finish(F, st);
+        conc = s->d_xfull;
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(s->d_pvol, pv.data(), np_all * sizeof(int), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(s->d_pk, pk.data(), np_all * sizeof(int), hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
-        e = ft::flux_planes<double>((int)np_all, s->d_vol, s->d_pvol, s->d_pk, s->d_coef, s->L.vs.x64, s->d_pout, st);
+        e = ft::flux_planes<double>((int)np_all, s->d_vol, s->d_pvol, s->d_pk, s->d_coef, conc, s->d_pout, st);
     if (e == hipSuccess) e = hipMemcpyAsync(planes.data(), s->d_pout, np_all * sizeof(double), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess)
         e = hipMemcpyAsync(zeros.data(), s->d_zero, s->max_active * sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
@@ -393,14 +551,19 @@ void worker_main(fast_server* s) {
                 return;
             }
         }
-        if (s->L.ntiles == 0) continue;
+        if (s->ntiles == 0) continue;
         // one round over the live set
         hipStream_t st = ctx->stream;
         hipError_t e = hipSuccess;
         for (int i = 0; i < m && e == hipSuccess; ++i) {
+            const bool upd = (i + 1) % kMixedUpdateEvery == 0 || i + 1 == m;
+            if (s->rb) {
+                e = ft::rb_iter(s->RL, st);
+                if (e == hipSuccess && upd) e = ft::rb_update(s->RL, st);
+                continue;
+            }
             e = (s->solver == FAST_BICGSTAB) ? Passes<float, double>::bicg_iter(s->L, st)
                                              : Passes<float, double>::cg_iter(s->L, st);
-            const bool upd = (i + 1) % kMixedUpdateEvery == 0 || i + 1 == m;
             if (e == hipSuccess && upd) e = ft::mixed_update(s->L, st);
             if (e == hipSuccess && upd) e = ft::mixed_residual(s->L, s->d_coef, st);
         }
@@ -439,17 +602,22 @@ int fast_server_create(fast_ctx* ctx, const fast_server_config* cfg, fast_server
     if (s->cfg.max_active <= 0) s->cfg.max_active = 64;
     if (s->cfg.iters_per_round <= 0) s->cfg.iters_per_round = 8;
     s->solver = s->cfg.use_bicgstab ? FAST_BICGSTAB : FAST_CG;
+    s->rb = (s->solver == FAST_CG);
     s->max_active = s->cfg.max_active;
     const size_t budget = s->cfg.hbm_budget ? s->cfg.hbm_budget : std::min(ctx->budget / 2, (size_t)16 << 30);
-    // per element: 8 FP32 planes + the FP64 solution
-    s->S = align_up(budget / (kPlanes * sizeof(float) + sizeof(double)), kAlign);
-    s->arena_mem = dalloc(ctx, s->S * (kPlanes * sizeof(float) + sizeof(double)));
+    // bytes per arena element: red-black, kRbBytes per padded per-colour element; full grid,
+    // 8 FP32 planes + the FP64 solution per voxel
+    const size_t per = s->rb ? kRbBytes : kPlanes * sizeof(float) + sizeof(double);
+    s->S = align_up(budget / per, kAlign);
+    if (s->S * per > budget && s->S > kAlign) s->S -= kAlign;
+    s->arena_mem = dalloc(ctx, s->S * per);
     s->d_vol = (VolDesc*)dalloc(ctx, sizeof(VolDesc) * s->max_active);
     s->d_coef = (Coef<double>*)dalloc(ctx, sizeof(Coef<double>) * s->max_active);
     s->d_st = (VolState<double>*)dalloc(ctx, sizeof(VolState<double>) * s->max_active);
     s->d_zero = (unsigned long long*)dalloc(ctx, sizeof(unsigned long long) * s->max_active);
     s->d_tab = (double*)dalloc(ctx, sizeof(double) * 2 * kTabEntries * s->max_active);
-    if (!s->arena_mem || !s->d_vol || !s->d_coef || !s->d_st || !s->d_zero || !s->d_tab ||
+    s->d_rb = (RbDesc*)dalloc(ctx, sizeof(RbDesc) * s->max_active);
+    if (!s->arena_mem || !s->d_vol || !s->d_coef || !s->d_st || !s->d_zero || !s->d_tab || !s->d_rb ||
         hipHostMalloc((void**)&s->h_st, sizeof(VolState<double>) * s->max_active, hipHostMallocDefault) != hipSuccess ||
         hipStreamCreateWithFlags(&s->up, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&s->up_done) != hipSuccess) {
         (void)hipGetLastError();
@@ -459,6 +627,7 @@ int fast_server_create(fast_ctx* ctx, const fast_server_config* cfg, fast_server
     s->arena.init(s->S);
     s->slots.assign(s->max_active, nullptr);
     s->h_vol.assign(s->max_active, VolDesc{});
+    s->h_rb.assign(s->max_active, RbDesc{});
     // every slot starts idle
     std::vector<VolState<double>> idle(s->max_active);
     for (auto& v : idle) v.status = ST_IDLE;
@@ -485,8 +654,14 @@ int fast_server_submit(fast_server* s, const uint8_t* mask, int nx, int ny, int 
     r->ny = ny;
     r->nz = nz;
     r->prm = *params;
-    r->elems = align_up(n, kAlign);
-    r->tz = choose_shape(nx, ny, nz, FAST_MIXED, s->ctx).tz;
+    if (s->rb) {   // padded per-colour elements
+        const int hx = (nx + 1) / 2;
+        r->elems = align_up((size_t)(hx + 2) * (size_t)(ny + 2) * (size_t)(nz + 2), kAlign);
+        r->tz = rb_tz(hx, ny, nz);
+    } else {
+        r->elems = align_up(n, kAlign);
+        r->tz = choose_shape(nx, ny, nz, FAST_MIXED, s->ctx).tz;
+    }
     {
         std::lock_guard<std::mutex> lk(s->mu);
         if (s->fatal != FAST_OK) {
@@ -563,6 +738,10 @@ void fast_server_destroy(fast_server* s) {
     dfree(ctx, s->d_pvol);
     dfree(ctx, s->d_pk);
     dfree(ctx, s->d_pout);
+    dfree(ctx, s->d_rb);
+    dfree(ctx, s->d_partb);
+    dfree(ctx, s->d_tile_fin);
+    dfree(ctx, s->d_xfull);
     if (s->h_st) (void)hipHostFree(s->h_st);
     if (s->up_done) (void)hipEventDestroy(s->up_done);
     if (s->up) (void)hipStreamDestroy(s->up);

@@ -826,7 +826,7 @@ struct RbBatch {
 };
 
 // Planes per RB tile: enough tiles to fill the chip (about 2048), 4..32 planes each.
-static int rb_tz(int hx, int ny, int nz) {
+int rb_tz(int hx, int ny, int nz) {
     if (const char* e = std::getenv("FAST_TAU_RB_TZ")) {
         const int v = std::atoi(e);
         if (v > 0) return v;

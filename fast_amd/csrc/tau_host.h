@@ -88,5 +88,7 @@ std::string shape_key(int nx, int ny, int nz, int precision);
 Shape choose_shape(int nx, int ny, int nz, int precision, const fast_ctx* ctx);
 void load_tuning(fast_ctx* ctx);
 int solve_jobs(fast_ctx* ctx, std::vector<Job>& jobs);
+// Planes per tile of the red-black passes for a volume with hx slots per row.
+int rb_tz(int hx, int ny, int nz);
 
 }  // namespace fth

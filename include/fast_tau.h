@@ -167,8 +167,10 @@ int fast_ccl_boundary_table(fast_volume* vol, uint8_t background, uint8_t* table
  * and leave on convergence.  No reference counterpart. */
 typedef struct fast_server fast_server;
 typedef struct fast_server_config {
-    size_t hbm_budget;      /* bytes of the Krylov-state arena, 40 B per voxel
-                             (0 = half the context budget, at most 16 GiB) */
+    size_t hbm_budget;      /* bytes of the Krylov-state arena (0 = half the context budget, at
+                             most 16 GiB); a request takes 46 B per element of its padded
+                             per-colour layout, (ceil(nx/2)+2)(ny+2)(nz+2) elements (about
+                             23 B per voxel), with CG; 40 B per voxel with BiCGSTAB */
     int max_active;         /* max concurrently solving requests (0 = 64) */
     int iters_per_round;    /* Krylov iterations between admission points (0 = 8) */
     int use_bicgstab;       /* 0: CG (default), 1: BiCGSTAB */
@@ -177,10 +179,12 @@ typedef struct fast_server_config {
 
 int fast_server_create(fast_ctx* ctx, const fast_server_config* cfg, fast_server** out);
 /* Requests are solved in the mixed form (FP32 vectors, FP64 arithmetic, reliable updates) to
- * params->tolerance on the FP64 true residual; params->solver and ->precision are ignored.
- * A request's result is bit-identical to fast_tau_solve_volume on the same volume with
- * precision FAST_MIXED and the server's solver, whatever else is in flight.  The server owns
- * its context while it runs: do not use the context from other threads meanwhile. */
+ * params->tolerance on the FP64 true residual: CG on the red-black Schur complement, or
+ * BiCGSTAB on the full grid (use_bicgstab); params->solver, ->precision and ->mixed_full_grid
+ * are ignored.  A request's result is bit-identical to fast_tau_solve_volume on the same volume
+ * with precision FAST_MIXED and the server's solver (mixed_full_grid = 0), whatever else is in
+ * flight.  The server owns its context while it runs: do not use the context from other
+ * threads meanwhile. */
 /* Copy the mask and queue a request; returns a ticket.  Never blocks on the GPU. */
 int fast_server_submit(fast_server* srv, const uint8_t* mask, int nx, int ny, int nz,
                        const fast_tau_params* params, uint64_t* ticket);

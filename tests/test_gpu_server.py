@@ -1,6 +1,6 @@
 """Continuous-batching server: every request's result is bit-identical to solving its volume
-alone in the mixed form on the full grid, whatever else was in flight, and admission respects
-the HBM budget."""
+alone in the mixed form (CG on the red-black Schur complement, or BiCGSTAB on the full grid),
+whatever else was in flight, and admission respects the HBM budget."""
 import numpy as np
 import pytest
 
@@ -28,7 +28,7 @@ def test_server_matches_solo(ctx, solver):
     solo = []
     for m, d in reqs:
         vol = ctx.volume(m)
-        res, _ = vol.solve(T.make_params(d, solver, "mixed", 1e-7, 10000, full_grid=True))
+        res, _ = vol.solve(T.make_params(d, solver, "mixed", 1e-7, 10000))
         vol.close()
         solo.append(res)
     # few slots and short rounds: requests join and leave while others are running
@@ -53,10 +53,12 @@ def test_server_matches_solo(ctx, solver):
 
 
 def test_server_budget_admission(ctx):
-    """An arena holding about two 32^3 volumes still serves eight of them, in FIFO admission,
-    and a request larger than the whole arena is refused at submit."""
-    per_elem = 8 * 4 + 8
-    with T.Server(ctx, hbm_budget=2 * 32 ** 3 * per_elem + 4096, max_active=8, iters_per_round=4) as srv:
+    """An arena holding about two and a half 32^3 volumes still serves eight of them, in FIFO
+    admission, and a request larger than the whole arena is refused at submit.  A red-black
+    request takes 46 B per element of its padded per-colour layout, (16 + 2) x 34 x 34
+    elements (64-aligned) at 32^3."""
+    per_request = 46 * ((18 * 34 * 34 + 63) // 64 * 64)
+    with T.Server(ctx, hbm_budget=int(2.5 * per_request), max_active=8, iters_per_round=4) as srv:
         masks = [F.generate_spheres(32, 3.0, 0.4, 100 + i) for i in range(8)]
         tickets = [srv.submit(m, T.make_params(4, CG, "mixed", 1e-6, 10000)) for m in masks]
         seen = []
