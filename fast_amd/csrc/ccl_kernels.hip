@@ -79,8 +79,9 @@ __global__ __launch_bounds__(256) void k_union(uint32_t* P, int nx, int ny, int 
     const long long plane = (long long)nx * ny;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         if (ld(P + i) == kNone) continue;
-        const int y = (int)((i / nx) % ny);
-        const int z = (int)(i / plane);
+        const unsigned row = (unsigned)i / (unsigned)nx;   // labels are 32-bit, so i < 2^32
+        const int y = (int)(row % (unsigned)ny);
+        const int z = (int)(row / (unsigned)ny);
         // one link per span and row pair is enough, but linking every voxel keeps this simple
         if (y > 0 && ld(P + i - nx) != kNone) merge(P, (uint32_t)i, (uint32_t)(i - nx));
         if (z > 0 && ld(P + i - plane) != kNone) merge(P, (uint32_t)i, (uint32_t)(i - plane));

@@ -30,6 +30,7 @@
 // bit.
 #include "tau_internal.h"
 
+#include <algorithm>
 #include <cfloat>
 
 namespace ft {
@@ -1156,36 +1157,66 @@ hipError_t launch_iter_cg(const Launch<T, A>& L, hipStream_t s, hipEvent_t* ev =
 }  // namespace
 
 // ---- mask preprocessing ----------------------------------------------------------------
+// Work item w = (x chunk of 256, group of 4 rows), row = y + ny z: a thread builds the codes of
+// one x in 4 rows.  Its 28 byte loads are unconditional and independent (a neighbour outside
+// the box is loaded from the voxel itself, which is the reference's clamp), so they are all in
+// flight at once; index arithmetic is 32-bit per work item (a 64-bit division per voxel made
+// this kernel ~100x slower than its bandwidth).  Zeros are counted per thread and added with
+// one atomic per block (atomics on one address serialise in L2).
 __global__ __launch_bounds__(256) void k_build_code(const uint8_t* __restrict__ mask, uint8_t* __restrict__ code,
                                                     int nx, int ny, int nz, unsigned long long* zero_count) {
-    const long long n = (long long)nx * ny * nz;
+    constexpr int R = 4;
+    const unsigned chunks = ((unsigned)nx + 255u) / 256u;
+    const unsigned rows = (unsigned)ny * (unsigned)nz;
+    const unsigned items = chunks * ((rows + R - 1) / R);
     const long long plane = (long long)nx * ny;
     unsigned long long zeros = 0;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-        const int x = (int)(i % nx);
-        const int y = (int)((i / nx) % ny);
-        const int z = (int)(i / plane);
-        const int own = mask[i] != 0 ? 1 : 0;
-        zeros += own ? 0ull : 1ull;
-        // clamped neighbours (Diffusion.cpp:51-62): a neighbour outside the box is the voxel itself
-        const int xp = (x + 1 < nx) ? (mask[i + 1] != 0) : own;
-        const int xm = (x > 0) ? (mask[i - 1] != 0) : own;
-        const int yp = (y + 1 < ny) ? (mask[i + nx] != 0) : own;
-        const int ym = (y > 0) ? (mask[i - nx] != 0) : own;
-        const int zp = (z + 1 < nz) ? (mask[i + plane] != 0) : own;
-        const int zm = (z > 0) ? (mask[i - plane] != 0) : own;
-        code[i] = (uint8_t)(xp | (xm << 1) | (yp << 2) | (ym << 3) | (zp << 4) | (zm << 5) | (own << 6));
+    for (unsigned w = blockIdx.x; w < items; w += gridDim.x) {
+        const unsigned rg = w / chunks;
+        const int x = (int)((w - rg * chunks) * 256u + threadIdx.x);
+        const int xc = min(x, nx - 1);
+        uint8_t m[R][7];
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const unsigned row = min(rg * R + j, rows - 1);
+            const int y = (int)(row % (unsigned)ny), z = (int)(row / (unsigned)ny);
+            const long long i = (long long)row * nx + xc;
+            m[j][0] = mask[i];
+            m[j][1] = mask[xc + 1 < nx ? i + 1 : i];
+            m[j][2] = mask[xc > 0 ? i - 1 : i];
+            m[j][3] = mask[y + 1 < ny ? i + nx : i];
+            m[j][4] = mask[y > 0 ? i - nx : i];
+            m[j][5] = mask[z + 1 < nz ? i + plane : i];
+            m[j][6] = mask[z > 0 ? i - plane : i];
+        }
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const unsigned row = rg * R + j;
+            if (x >= nx || row >= rows) continue;
+            const int own = m[j][0] != 0;
+            zeros += own ? 0ull : 1ull;
+            code[(long long)row * nx + x] = (uint8_t)((m[j][1] != 0) | ((m[j][2] != 0) << 1) | ((m[j][3] != 0) << 2) |
+                                                      ((m[j][4] != 0) << 3) | ((m[j][5] != 0) << 4) |
+                                                      ((m[j][6] != 0) << 5) | (own << 6));
+        }
     }
     zeros = wave_sum(zeros);
-    if ((threadIdx.x & 63) == 0 && zeros) atomicAdd(zero_count, zeros);
+    __shared__ unsigned long long wz[4];
+    if ((threadIdx.x & 63) == 0) wz[threadIdx.x >> 6] = zeros;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long t = wz[0] + wz[1] + wz[2] + wz[3];
+        if (t) atomicAdd(zero_count, t);
+    }
 }
 
 hipError_t build_code(const uint8_t* mask, uint8_t* code, int nx, int ny, int nz, unsigned long long* zero_count,
                       hipStream_t s) {
-    const long long n = (long long)nx * ny * nz;
-    long long blocks = (n + 255) / 256;
-    if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(k_build_code, dim3((unsigned)blocks), dim3(256), 0, s, mask, code, nx, ny, nz, zero_count);
+    const unsigned long long rows = (unsigned long long)ny * nz;
+    const unsigned long long items = (((unsigned long long)nx + 255) / 256) * ((rows + 3) / 4);
+    if (rows > 0xFFFFFFFFull || items > 0xFFFFFFFFull) return hipErrorInvalidValue;
+    const unsigned blocks = (unsigned)std::min<unsigned long long>(items, 2048);
+    hipLaunchKernelGGL(k_build_code, dim3(blocks), dim3(256), 0, s, mask, code, nx, ny, nz, zero_count);
     return hipGetLastError();
 }
 
@@ -1558,9 +1589,9 @@ __global__ __launch_bounds__(256) void k_flux_planes(const VolDesc* vol, const i
     const long long sb = ib == 1 ? nx : plane;
     AccDD acc[1];
     acc[0].init();
-    for (long long q = threadIdx.x; q < n; q += 256) {
-        const long long a = q % na, b = q / na;
-        const long long lin = (long long)k * stride_p + a * sa + b * sb;
+    for (unsigned q = threadIdx.x; q < (unsigned)n; q += 256) {   // a plane has < 2^32 voxels
+        const unsigned a = q % (unsigned)na, b = q / (unsigned)na;
+        const long long lin = (long long)k * stride_p + (long long)a * sa + (long long)b * sb;
         const int own = (vd.code[lin] >> 6) & 1;
         const T D = own ? cf.dflux1 : cf.dflux0;
         const T c = conc[vd.off + lin];
