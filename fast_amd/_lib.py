@@ -46,7 +46,8 @@ class TauParams(C.Structure):
         ("mixed_outer_tol", C.c_double),
         ("verbose", C.c_int),
         ("mixed_full_grid", C.c_int),
-        ("reserved", C.c_int * 2),
+        ("flux_nonuniform", C.c_int),
+        ("reserved", C.c_int * 1),
     ]
 
 

@@ -318,6 +318,7 @@ int rebuild(fast_server* s, const std::vector<Req*>& adm) {
         d.tile_base = tiles;
         d.ntiles = d.tiles_x * d.tiles_y * d.tiles_z;
         d.plane_base = 0;
+        d.flux_nu = r->prm.flux_nonuniform ? 1 : 0;
         d.off = (long long)r->off;
         d.code = r->d_code;
         for (int t = 0; t < d.ntiles; ++t) tv.push_back(i);
@@ -392,6 +393,7 @@ int retire(fast_server* s, const std::vector<int>& fin) {
             v.dir = d.dir;
             v.primary = d.primary;
             v.sgn = d.sgn;
+            v.flux_nu = s->slots[i]->prm.flux_nonuniform ? 1 : 0;
             v.off = (long long)xn;
             v.code = s->slots[i]->d_code;
             d.off = (long long)xn;

@@ -484,6 +484,7 @@ static int prepare(Batch<T, A>& B) {
         d.tile_base = tiles;
         d.ntiles = d.tiles_x * d.tiles_y * d.tiles_z;
         d.plane_base = planes;
+        d.flux_nu = j.prm.flux_nonuniform ? 1 : 0;
         d.off = (long long)total;
         d.code = v->d_code;
         tiles += d.ntiles;
@@ -883,6 +884,7 @@ static int prepare_rb(RbBatch& B) {
         d.sgn = r.sgn;
         d.tiles_x = d.tiles_y = d.tiles_z = d.tz = 1;
         d.plane_base = planes;
+        d.flux_nu = j.prm.flux_nonuniform ? 1 : 0;
         d.off = r.off;
         d.code = v->d_code;
         const int np = d.primary == 0 ? d.nx : (d.primary == 1 ? d.ny : d.nz);

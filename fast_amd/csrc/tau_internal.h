@@ -31,7 +31,8 @@ struct VolDesc {
     int dir, primary, sgn;
     int tiles_x, tiles_y, tiles_z, tz;  // tile grid; tz = planes per tile (z chunk)
     int tile_base, ntiles;              // range in the launch's tile list
-    int plane_base, pad;                // flux: first plane slot of this volume
+    int plane_base;                     // flux: first plane slot of this volume
+    int flux_nu;                        // flux: TortuosityParams::useNonUniform
     long long off;                      // element offset in the concatenated vectors
     const uint8_t* code;                // per-voxel stencil code (bits 0-5 nbr solid, bit 6 own solid)
 };

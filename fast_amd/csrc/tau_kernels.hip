@@ -1587,25 +1587,40 @@ __global__ __launch_bounds__(256) void k_flux_planes(const VolDesc* vol, const i
     const int na = dims[ia];
     const long long sa = ia == 0 ? 1 : nx;
     const long long sb = ib == 1 ? nx : plane;
+    const bool nu = vd.flux_nu != 0;
     AccDD acc[1];
     acc[0].init();
+    unsigned pores = 0;
     for (unsigned q = threadIdx.x; q < (unsigned)n; q += 256) {   // a plane has < 2^32 voxels
         const unsigned a = q % (unsigned)na, b = q / (unsigned)na;
         const long long lin = (long long)k * stride_p + (long long)a * sa + (long long)b * sb;
-        const int own = (vd.code[lin] >> 6) & 1;
+        const int code = vd.code[lin];
+        const int own = (code >> 6) & 1;
+        pores += own ? 0u : 1u;
+        if (own && nu) continue;   // non-uniform mode: pore voxels only
         const T D = own ? cf.dflux1 : cf.dflux0;
         const T c = conc[vd.off + lin];
         T cn = T(0), dx = T(0.5);
         if (has_next) {
             cn = conc[vd.off + lin + step];
             dx = T(1.0);
+            // non-uniform mode: half spacing towards a solid neighbour (the code's solid flag of
+            // the neighbour in the flux direction: bit 0 for +x, 1 for -x, ... in Dir order)
+            if (nu && ((code >> vd.dir) & 1)) dx = T(0.5);
         }
         const T term = D * (c - cn) / dx;
         acc[0].add2((double)term, 0.0);
     }
     double sum[2] = {0.0, 0.0};
     block_reduce_store<AccDD, 1, 256>(acc, sum);   // thread 0 receives the plane sum (hi, lo)
-    if (threadIdx.x == 0) plane_out[gp] = (double)(T(sum[0] + sum[1]) / T(n));
+    pores = wave_sum(pores);
+    __shared__ unsigned wp[4];
+    if ((threadIdx.x & 63) == 0) wp[threadIdx.x >> 6] = pores;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long pc = (unsigned long long)wp[0] + wp[1] + wp[2] + wp[3];
+        plane_out[gp] = (double)(T(sum[0] + sum[1]) / (nu ? T(pc) : T(n)));
+    }
 }
 }  // namespace
 

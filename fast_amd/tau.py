@@ -243,7 +243,7 @@ class Volume:
 
 def make_params(d=Dir.X_POS, solver=DiffusionSolverType.DSOLVER_BICGSTAB, precision="double", tol=1e-6,
                 max_iter=10000, coeffs=(1.0, 0.001), porosity=-1.0, inner_tol=0.0, outer_tol=0.0,
-                verbose=False, full_grid=False) -> TauParams:
+                verbose=False, full_grid=False, nonuniform=False) -> TauParams:
     p = TauParams()
     lib().fast_tau_params_default(C.byref(p))
     p.dir = int(d)
@@ -258,6 +258,7 @@ def make_params(d=Dir.X_POS, solver=DiffusionSolverType.DSOLVER_BICGSTAB, precis
     p.mixed_outer_tol = float(outer_tol)
     p.verbose = int(bool(verbose))
     p.mixed_full_grid = int(bool(full_grid))
+    p.flux_nonuniform = int(bool(nonuniform))
     return p
 
 
@@ -288,13 +289,11 @@ def getTortuosity(mask, params: TortuosityParams, solverType=DiffusionSolverType
     iteration count is appended to `iterationsOut` when given."""
     if not params.onDevice:
         raise NotImplementedError("this library has no CPU solver (onDevice=False)")
-    if params.useNonUniform:
-        raise NotImplementedError("the non-uniform (reachable-only) solve is disabled in the reference CLI")
     ctx = ctx or (mask.ctx if isinstance(mask, Volume) else default_context())
     vol = mask if isinstance(mask, Volume) else Volume(ctx, mask)
     por = params.porosity if params.porosityPrecomputed else -1.0
     p = make_params(params.dir, solverType, precision, params.tolerance, params.maxIter, params.coeffs, por,
-                    params.mixed_inner_tol, params.mixed_outer_tol, params.verbose)
+                    params.mixed_inner_tol, params.mixed_outer_tol, params.verbose, nonuniform=params.useNonUniform)
     try:
         res, conc = vol.solve(p, conc=concetrationOutput is not None)
     except FastError:

@@ -66,7 +66,10 @@ typedef struct fast_tau_params {
     int verbose;
     int mixed_full_grid;  /* MIXED + CG only: 0 (default) = CG on the red-black Schur complement
                              (about half the iterations); 1 = CG on the full grid */
-    int reserved[2];
+    int flux_nonuniform;  /* TortuosityParams::useNonUniform (VolumeMeasures.h:27): the flux sums
+                             pore voxels only, with half spacing towards a solid neighbour, and a
+                             plane's mean divides by its pore count (VolumeMeasures.cpp:228-264) */
+    int reserved[1];
 } fast_tau_params;
 
 /* Output of one solve: getTortuosity's return value and iterationsOut, plus status. */

@@ -56,9 +56,9 @@ def lib():
         h.oracle_invdiag_f64.argtypes = [u8, C.c_int, C.c_int, C.c_int, C.c_int, dp]
         h.oracle_tau.restype = C.c_int
         h.oracle_tau.argtypes = [u8, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_long,
-                                 C.c_int, C.c_double, dp, C.POINTER(OracleResult)]
+                                 C.c_int, C.c_double, dp, C.POINTER(OracleResult), C.c_int]
         h.oracle_flux_tau_f64.restype = C.c_double
-        h.oracle_flux_tau_f64.argtypes = [u8, C.c_int, C.c_int, C.c_int, C.c_int, dp, C.c_double]
+        h.oracle_flux_tau_f64.argtypes = [u8, C.c_int, C.c_int, C.c_int, C.c_int, dp, C.c_double, C.c_int]
         h.oracle_ccl.restype = C.c_uint32
         h.oracle_ccl.argtypes = [u8, C.c_int, C.c_int, C.c_int, C.c_uint8, u32]
         h.oracle_boundary_table.restype = None
@@ -120,21 +120,22 @@ def invdiag(mask, d):
     return out
 
 
-def tau(mask, d, solver=0, precision=0, tol=1e-6, max_iter=10000, threads=0, porosity=-1.0, conc=False):
+def tau(mask, d, solver=0, precision=0, tol=1e-6, max_iter=10000, threads=0, porosity=-1.0, conc=False,
+        nonuniform=False):
     """Restated getTortuosity<T> on the CPU.  solver 0 = BiCGSTAB, 1 = CG; precision 0 = double,
-    1 = float.  Returns (OracleResult, conc or None)."""
+    1 = float; nonuniform = TortuosityParams::useNonUniform.  Returns (OracleResult, conc or None)."""
     m, p, nx, ny, nz = _m(mask)
     r = OracleResult()
     c = np.empty(m.shape, dtype=np.float64) if conc else None
     lib().oracle_tau(p, nx, ny, nz, int(d), int(solver), int(precision), float(tol), int(max_iter), int(threads),
-                     float(porosity), _dp(c) if conc else None, C.byref(r))
+                     float(porosity), _dp(c) if conc else None, C.byref(r), int(bool(nonuniform)))
     return r, c
 
 
-def flux_tau(mask, d, c, porosity):
+def flux_tau(mask, d, c, porosity, nonuniform=False):
     m, p, nx, ny, nz = _m(mask)
     cc = np.ascontiguousarray(c, dtype=np.float64)
-    return float(lib().oracle_flux_tau_f64(p, nx, ny, nz, int(d), _dp(cc), float(porosity)))
+    return float(lib().oracle_flux_tau_f64(p, nx, ny, nz, int(d), _dp(cc), float(porosity), int(bool(nonuniform))))
 
 
 def ccl(mask, background=255):

@@ -26,6 +26,9 @@
  *                       <RowMajor, Lower|Upper> (Eigen 3.3.x ConjugateGradient.h,
  *                       restated below; GPU mirror fastlib/src/cuda/CG.cu:20-171)
  *   - flux + tau        fastlib/src/volume/VolumeMeasures.cpp:157-159,188-280
+ *                       (useNonUniform: :228-231 pore voxels only, :242-245 half
+ *                       spacing towards a solid neighbour, :261-264 plane mean
+ *                       over the pore count)
  *   - reachable pores   fastlib/src/cuda/VolumeCCL.cu:86-170 (toggle spans),
  *                       :204-298 (6-connectivity by span overlap), :326-375,547
  *                       (labels renumbered by the component's minimum linear
@@ -421,9 +424,9 @@ SolveOut cg(const CSR<T>& A, std::vector<T>& x, T tol, long maxIters) {
     return o;
 }
 
-// Flux integral and tau (VolumeMeasures.cpp:188-280), useNonUniform = false.
+// Flux integral and tau (VolumeMeasures.cpp:188-280); nonuniform = useNonUniform.
 template <class T>
-T fluxTau(const uint8_t* mask, const Dims& d, int dir, const T* c, T porosity, double* dcOut) {
+T fluxTau(const uint8_t* mask, const Dims& d, int dir, const T* c, T porosity, double* dcOut, bool nonuniform) {
     const int primary = dirIndex(dir);
     const int sec[2] = {(primary + 1) % 3, (primary + 2) % 3};
     const int n = d.r[sec[0]] * d.r[sec[1]];
@@ -433,6 +436,7 @@ T fluxTau(const uint8_t* mask, const Dims& d, int dir, const T* c, T porosity, d
     T dcSumOverPlanes = 0;
     for (int k = 0; k < d.r[primary]; ++k) {
         DD sumD;   // order-independent plane sum (the reference sums sequentially in T)
+        size_t poreCount = 0;
         for (int j = 0; j < d.r[sec[1]]; ++j) {
             for (int i = 0; i < d.r[sec[0]]; ++i) {
                 int pos[3];
@@ -440,6 +444,8 @@ T fluxTau(const uint8_t* mask, const Dims& d, int dir, const T* c, T porosity, d
                 pos[sec[0]] = i;
                 pos[sec[1]] = j;
                 const size_t vi = d.lin(pos[0], pos[1], pos[2]);
+                if (mask[vi] == 0) poreCount++;
+                if (mask[vi] != 0 && nonuniform) continue;   // :228-231
                 const T coeffHere = coeffs[(mask[vi] == 0) ? 0 : 1];
                 const T concHere = c[vi];
                 T concPrev = T(0.0);
@@ -448,12 +454,13 @@ T fluxTau(const uint8_t* mask, const Dims& d, int dir, const T* c, T porosity, d
                 if (q[0] >= 0 && q[1] >= 0 && q[2] >= 0 && q[0] < d.r[0] && q[1] < d.r[1] && q[2] < d.r[2]) {
                     concPrev = c[d.lin(q[0], q[1], q[2])];
                     concDx = T(1.0);
+                    if (nonuniform && mask[d.lin(q[0], q[1], q[2])] != 0) concDx = T(0.5);   // :242-245
                 }
                 const T term = coeffHere * (concHere - concPrev) / concDx;
                 sumD.add2((double)term, 0.0);
             }
         }
-        const T dcPlane = T(sumD.value()) / n;
+        const T dcPlane = nonuniform ? T(sumD.value()) / T(poreCount) : T(sumD.value()) / n;   // :260-264
         dcSumOverPlanes += dcPlane;
     }
     const T dc = dcSumOverPlanes / d.r[primary];
@@ -566,7 +573,7 @@ void oracle_invdiag_f64(const uint8_t* mask, int nx, int ny, int nz, int dir, do
  * (used only to time a bounded sample; the result is then not a converged tau). */
 int oracle_tau(const uint8_t* mask, int nx, int ny, int nz, int dir, int solver, int precision,
                double tol, long max_iter, int threads, double porosity, double* conc_out,
-               oracle_result* res) {
+               oracle_result* res, int nonuniform) {
     std::memset(res, 0, sizeof(*res));
     if (!mask || nx <= 0 || ny <= 0 || nz <= 0 || dir < 0 || dir > 5 || solver < 0 || solver > 1) {
         res->status = 2;
@@ -593,7 +600,7 @@ int oracle_tau(const uint8_t* mask, int nx, int ny, int nz, int dir, int solver,
             return;
         }
         double dc = 0;
-        const T tau = fluxTau<T>(mask, d, dir, x.data(), T((double)por), &dc);
+        const T tau = fluxTau<T>(mask, d, dir, x.data(), T((double)por), &dc, nonuniform != 0);
         res->dc = dc;
         res->tau = (double)tau;
         res->status = 0;
@@ -606,9 +613,10 @@ int oracle_tau(const uint8_t* mask, int nx, int ny, int nz, int dir, int solver,
 }
 
 /* Flux/tau of a given FP64 concentration field (flux formula alone). */
-double oracle_flux_tau_f64(const uint8_t* mask, int nx, int ny, int nz, int dir, const double* c, double porosity) {
+double oracle_flux_tau_f64(const uint8_t* mask, int nx, int ny, int nz, int dir, const double* c, double porosity,
+                           int nonuniform) {
     Dims d{{nx, ny, nz}};
-    return fluxTau<double>(mask, d, dir, c, porosity, nullptr);
+    return fluxTau<double>(mask, d, dir, c, porosity, nullptr, nonuniform != 0);
 }
 
 /* Connected-component labels of the span set (6-connected), renumbered 1..K by the

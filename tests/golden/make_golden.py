@@ -7,7 +7,10 @@ tiny_tau.json  -- for small masks (<= 36 voxels) and every direction: the exact 
                   face D = mean of the two cells, half-cell Dirichlet faces on the primary axis,
                   no flux through lateral box faces; VolumeMeasures.cpp:188-280 flux with the
                   cell's own D; h = float(1/maxDim); D = float(1.0), float(0.001)) with exact
-                  rationals -- no Krylov iteration, no rounding.
+                  rationals -- no Krylov iteration, no rounding.  tau_nu is the same solution's
+                  tau with useNonUniform (VolumeMeasures.cpp:228-264: pore cells only, half
+                  spacing towards a solid neighbour, plane mean over the plane's pore count;
+                  "nan" when a plane has no pore cell, as 0/0 in the reference).
 ccl_cases.json -- reachable-pore labels from a literal restatement of the reference's span
                   algorithm (VolumeCCL.cu:86-375: toggle spans per x-row, overlap links between
                   neighbouring rows, minimum span-start label, renumbering in row-major root
@@ -85,8 +88,12 @@ def exact_tau(mask: np.ndarray, d: int):
     sec = [(k + 1) % 3, (k + 2) % 3]
     n = dims[sec[0]] * dims[sec[1]]
     dc_sum = Fraction(0)
+    nu_sum = Fraction(0)
+    nu_nan = False
     for kk in range(Np):
         s_plane = Fraction(0)
+        s_nu = Fraction(0)
+        pores = 0
         for j in range(dims[sec[1]]):
             for i in range(dims[sec[0]]):
                 p = [0, 0, 0]
@@ -94,21 +101,38 @@ def exact_tau(mask: np.ndarray, d: int):
                 p[sec[0]] = i
                 p[sec[1]] = j
                 own = 1 if mask[p[2], p[1], p[0]] else 0
+                pores += 1 - own
                 c = sol[idx(*p)]
                 q = list(p)
                 q[k] += sgn
                 if 0 <= q[k] < dims[k]:
                     s_plane += D[own] * (c - sol[idx(*q)])
+                    nb = 1 if mask[q[2], q[1], q[0]] else 0
+                    if not own:
+                        s_nu += D[own] * (c - sol[idx(*q)]) / (Fraction(1, 2) if nb else 1)
                 else:
                     s_plane += D[own] * c / Fraction(1, 2)
+                    if not own:
+                        s_nu += D[own] * c / Fraction(1, 2)
         dc_sum += s_plane / n
+        if pores:
+            nu_sum += s_nu / pores
+        else:
+            nu_nan = True
     dc = dc_sum / Np
     zeros = int((mask == 0).sum())
     phi = Fraction(float(np.float32(zeros / N)))
+    # a plane without pore cells gives 0/0; zero flux gives phi/0 (inf, or 0/0 with no pores)
+    if nu_nan or (nu_sum == 0 and phi == 0):
+        tau_nu = float("nan")
+    elif nu_sum == 0:
+        tau_nu = float("inf")
+    else:
+        tau_nu = phi / nu_sum
     if dc == 0:
-        return float("inf"), sol   # zero rhs: c = 0, no flux (the reference divides by zero too)
+        return float("inf"), tau_nu, sol   # zero rhs: c = 0, no flux (the reference divides by zero too)
     tau = phi / (dc * Np)
-    return tau, sol
+    return tau, tau_nu, sol
 
 
 def tiny_masks():
@@ -229,10 +253,11 @@ def main():
     recs = []
     for name, m in tiny_masks():
         for d in range(6):
-            tau, sol = exact_tau(m, d)
+            tau, tau_nu, sol = exact_tau(m, d)
             exact = f"{tau.numerator}/{tau.denominator}" if isinstance(tau, Fraction) else "inf"
+            exact_nu = f"{tau_nu.numerator}/{tau_nu.denominator}" if isinstance(tau_nu, Fraction) else str(tau_nu)
             recs.append({"name": name, "shape_zyx": list(m.shape), "mask": m.ravel().tolist(), "dir": d,
-                         "tau": float(tau), "tau_exact": exact,
+                         "tau": float(tau), "tau_exact": exact, "tau_nu": float(tau_nu), "tau_nu_exact": exact_nu,
                          "conc": [float(v) for v in sol]})
     (HERE / "tiny_tau.json").write_text(json.dumps(recs))
     crecs = []

@@ -194,3 +194,51 @@ def test_get_tortuosity_mirror(ctx):
                           iterationsOut=its, ctx=ctx)
     ref, _ = O.tau(m, 3, solver=CG, tol=1e-6)
     assert tau == ref.tau and its[0] == ref.iterations
+
+
+NU_CASES = [OPERATOR_CASES[0], OPERATOR_CASES[1], OPERATOR_CASES[2], OPERATOR_CASES[6]]
+
+
+@pytest.mark.parametrize("name,make", NU_CASES, ids=[c[0] for c in NU_CASES])
+def test_nonuniform_flux_bit_identical(ctx, name, make):
+    """useNonUniform flux (VolumeMeasures.cpp:228-264): FP64 and FP32 tau equal the oracle's bit
+    for bit (same solution, same plane sums); the mixed red-black solve at 1e-11 is within 1e-7
+    of the oracle's converged value."""
+    m = make()
+    vol = ctx.volume(m)
+    try:
+        for d in (0, 3, 4):
+            for prec, pi in (("double", 0), ("float", 1)):
+                for solver in (BICG, CG):
+                    ref, _ = O.tau(m, d, solver=solver, precision=pi, tol=1e-6, nonuniform=True)
+                    res, _ = vol.solve(T.make_params(d, solver, prec, 1e-6, 10000, nonuniform=True))
+                    assert res.iterations == ref.iterations, (name, d, prec, solver)
+                    if np.isnan(ref.tau):
+                        assert np.isnan(res.tau), (name, d, prec, solver, res.tau)
+                    else:
+                        assert res.tau == ref.tau, (name, d, prec, solver, res.tau, ref.tau)
+            tight, _ = O.tau(m, d, solver=CG, tol=1e-13, nonuniform=True)
+            mixed, _ = vol.solve(T.make_params(d, CG, "mixed", 1e-11, 20000, nonuniform=True))
+            if np.isnan(tight.tau):
+                assert np.isnan(mixed.tau)
+            else:
+                assert abs(mixed.tau - tight.tau) <= 1e-7 * abs(tight.tau), (name, d, mixed.tau, tight.tau)
+    finally:
+        vol.close()
+
+
+def test_nonuniform_tiny_golden(ctx):
+    """The exact-rational tau_nu of the tiny systems (tests/golden/tiny_tau.json)."""
+    recs = json.loads((GOLD / "tiny_tau.json").read_text())
+    for r in recs:
+        m = np.array(r["mask"], dtype=np.uint8).reshape(r["shape_zyx"])
+        vol = ctx.volume(m)
+        res, _ = vol.solve(T.make_params(r["dir"], CG, "double", 1e-13, 10000, nonuniform=True))
+        vol.close()
+        want = r["tau_nu"]
+        if np.isnan(want):
+            assert np.isnan(res.tau), (r["name"], r["dir"])
+        elif np.isinf(want):
+            assert np.isinf(res.tau), (r["name"], r["dir"])
+        else:
+            assert abs(res.tau - want) <= 1e-10 * want, (r["name"], r["dir"], res.tau, want)

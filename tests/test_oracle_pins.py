@@ -179,3 +179,46 @@ def test_porosity_counts_all_pores_as_float():
     m = _vol(5, shape=(13, 17, 19))
     assert O.count_zero(m) == int((m == 0).sum())
     assert O.porosity(m) == float(np.float32((m == 0).sum() / m.size))
+
+
+@pytest.mark.parametrize("solver", [0, 1])
+def test_nonuniform_flux_exact_rational(solver):
+    """useNonUniform (VolumeMeasures.cpp:228-264) on the exact solutions of the tiny systems:
+    tau_nu from the rational generator, nan where a plane has no pore cell, inf for zero flux."""
+    for r in _tiny():
+        m = np.array(r["mask"], dtype=np.uint8).reshape(r["shape_zyx"])
+        res, _ = O.tau(m, r["dir"], solver=solver, tol=1e-13, nonuniform=True)
+        want = r["tau_nu"]
+        if np.isnan(want):
+            assert np.isnan(res.tau), (r["name"], r["dir"], res.tau)
+        elif np.isinf(want):
+            assert np.isinf(res.tau), (r["name"], r["dir"], res.tau)
+        else:
+            assert abs(res.tau - want) <= 1e-10 * want, (r["name"], r["dir"], res.tau, want)
+
+
+def test_nonuniform_flux_closed_forms():
+    """Parallel slabs: every pore cell carries the same flux 1/N, so the pore-count mean is 1/N
+    and tau_nu = float(phi).  A fully solid plane across the flow has no pore cell: 0/0.
+    All pore: no solid cells, so tau_nu = tau = 1."""
+    m = np.zeros((30, 27, 45), np.uint8)
+    m[:, ::3, :] = 255
+    phi_f = float(np.float32((m == 0).mean()))
+    for solver in (0, 1):
+        res, _ = O.tau(m, 0, solver=solver, tol=1e-12, nonuniform=True)
+        assert abs(res.tau - phi_f) < 1e-9 * phi_f
+    blocked = np.zeros((8, 6, 10), np.uint8)
+    blocked[:, :, 4] = 255
+    res, _ = O.tau(blocked, 0, solver=1, tol=1e-12, nonuniform=True)
+    assert np.isnan(res.tau)
+    res, _ = O.tau(np.zeros((8, 6, 10), np.uint8), 2, solver=1, tol=1e-12, nonuniform=True)
+    assert abs(res.tau - 1.0) < 1e-12
+
+
+def test_nonuniform_flip_invariance():
+    m = _vol(4)
+    t_pos, _ = O.tau(m, 0, tol=1e-12, nonuniform=True)
+    t_neg, _ = O.tau(m[:, :, ::-1], 1, tol=1e-12, nonuniform=True)
+    assert abs(t_pos.tau - t_neg.tau) < 1e-9 * t_pos.tau
+    t_uni, _ = O.tau(m, 0, tol=1e-12)
+    assert abs(t_uni.tau - t_pos.tau) > 1e-3 * t_pos.tau   # the two flux modes differ
