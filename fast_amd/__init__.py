@@ -3,12 +3,12 @@
 The compute path is libfasttau.so (hand-written HIP kernels for gfx950 behind a C ABI,
 include/fast_tau.h); this package is its host-side mirror of the reference interface.
 """
-from .tau import (Context, DiffusionSolverType, Dir, Precision, Server, TortuosityParams, Volume,
-                  generate_ellipsoids, generate_spheres, getDirs, getPorosity, getTortuosity, load_bin, make_params,
-                  save_bin, solve_batch)
+from .tau import (Context, DiffusionSolverType, Dir, Precision, Server, TortuosityParams, Volume, binarize,
+                  generate_ellipsoids, generate_spheres, getDirs, getPorosity, getSubvolume, getTortuosity, load_bin,
+                  loadTiffFolder, make_params, rasterizeSpheres, save_bin, solve_batch)
 
 __all__ = [
-    "Context", "DiffusionSolverType", "Dir", "Precision", "Server", "TortuosityParams", "Volume", "generate_ellipsoids",
-    "generate_spheres", "getDirs", "getPorosity", "getTortuosity", "load_bin", "make_params", "save_bin",
-    "solve_batch",
+    "Context", "DiffusionSolverType", "Dir", "Precision", "Server", "TortuosityParams", "Volume", "binarize",
+    "generate_ellipsoids", "generate_spheres", "getDirs", "getPorosity", "getSubvolume", "getTortuosity", "load_bin",
+    "loadTiffFolder", "make_params", "rasterizeSpheres", "save_bin", "solve_batch",
 ]

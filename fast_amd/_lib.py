@@ -112,6 +112,12 @@ PROTOTYPES = [
     ("fast_server_destroy", None, [_vp]),
     ("fast_load_bin_dims", C.c_int, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("fast_load_bin", C.c_int, [C.c_char_p, _u8p, C.c_size_t]),
+    ("fast_tiff_folder_dims", C.c_int, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    ("fast_load_tiff_folder", C.c_int, [C.c_char_p, _u8p, C.c_size_t]),
+    ("fast_binarize", None, [_u8p, C.c_size_t, C.c_float]),
+    ("fast_rasterize_sph", C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, _u8p]),
+    ("fast_crop", C.c_int, [_u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                            _u8p]),
     ("fast_save_bin", C.c_int, [C.c_char_p, _u8p, C.c_int, C.c_int, C.c_int]),
     ("fast_generate_spheres", C.c_long, [_u8p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_uint64]),
     ("fast_generate_ellipsoids", C.c_long, [_u8p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64]),

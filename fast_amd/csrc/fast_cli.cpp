@@ -4,13 +4,19 @@
 // fast/src/VolumeIOModule.h, fast/src/CSVOutputModule.h): the same flags, the same
 // direction strings (fast/src/utility.h:84-113), one CSV row per direction with the same
 // key-sorted columns and default 6-significant-digit formatting, header only when the
-// output is new.  Additions: `-p mixed` (FP32 stencil / FP64 refinement), `--reach FILE`
-// (reachable-pore mask of the first direction's axis as .bin) and `--gpu N`.
-// `--cpu` is not offered: this library has no CPU solver.
+// output is new.  Inputs as loadVolume (fast/src/VolumeIOModule.h:41-131): a `.bin` file, a
+// folder of TIFF slices (binarized with threshold 1), or a `.sph` sphere list rasterized at
+// the --sub resolution; then the --sub/--origin, --randsub and --aspect crops
+// (VolumeIOModule.h:153-245).  Additions: `-p mixed` (FP32 storage / FP64 arithmetic),
+// `--reach FILE` (reachable-pore mask of the first direction's axis as .bin) and `--gpu N`.
+// `--cpu` is not offered: this library has no CPU solver; `.pos` inputs are not read.
 #include "fast_tau.h"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <random>
+#include <sys/stat.h>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -84,8 +90,43 @@ private:
     int rows_ = 0;
 };
 
+// "x" or "x,y,z" (the reference's ivec3/vec3 flag values)
+template <class T>
+bool parse3(const std::string& s, T (&v)[3]) {
+    std::stringstream ss(s);
+    std::string part;
+    int n = 0;
+    while (std::getline(ss, part, ',')) {
+        if (n == 3) return false;
+        std::istringstream ps(part);
+        if (!(ps >> v[n])) return false;
+        ++n;
+    }
+    if (n == 1) v[1] = v[2] = v[0];
+    return n == 1 || n == 3;
+}
+
+bool is_directory(const std::string& p) {
+    struct stat st;
+    return stat(p.c_str(), &st) == 0 && S_ISDIR(st.st_mode);
+}
+
+std::string extension(const std::string& p) {   // checkExtension: text after the last '.'
+    const size_t dot = p.find_last_of('.');
+    return dot == std::string::npos ? p : p.substr(dot + 1);
+}
+
+bool crop(std::vector<uint8_t>& m, int (&dim)[3], const int (&o)[3], const int (&sz)[3]) {
+    std::vector<uint8_t> out((size_t)sz[0] * sz[1] * sz[2]);
+    if (fast_crop(m.data(), dim[0], dim[1], dim[2], o[0], o[1], o[2], sz[0], sz[1], sz[2], out.data()) != FAST_OK)
+        return false;
+    m.swap(out);
+    for (int a = 0; a < 3; ++a) dim[a] = sz[a];
+    return true;
+}
+
 void usage() {
-    std::cout << "Usage: fast tau <input.bin> [options]\n"
+    std::cout << "Usage: fast tau <input> [options]   (input: .bin file, folder of .tif/.tiff slices, .sph)\n"
                  "  -d, --dir STR        x|y|z (combinable), all, pos, neg, xneg, yneg, zneg (default x-)\n"
                  "  --solver CG|BICGSTAB  (default CG)\n"
                  "  --tol N              tolerance 1e-N (default 6)\n"
@@ -95,7 +136,11 @@ void usage() {
                  "  -v, --verbose\n"
                  "  --over, --oversubscribe  accepted, no effect (288 GB HBM)\n"
                  "  --reach FILE         write the reachable-pore mask (.bin) of the first direction\n"
-                 "  --gpu N              HIP device (default 0)\n";
+                 "  --gpu N              HIP device (default 0)\n"
+                 "  --sub X|X,Y,Z        subvolume size (and the .sph resolution)\n"
+                 "  --origin X|X,Y,Z     subvolume origin\n"
+                 "  --randsub X|X,Y,Z    subvolume of this size at a random origin\n"
+                 "  --aspect X|X,Y,Z     aspect-ratio crop; one component must be 1.0\n";
 }
 
 }  // namespace
@@ -119,6 +164,9 @@ int main(int argc, char** argv) {
     int tol = 6, gpu = 0;
     long iters = 10000;
     bool verbose = false;
+    int sub[3] = {0, 0, 0}, origin[3] = {0, 0, 0}, randsub[3] = {0, 0, 0};
+    float aspect[3] = {1.0f, 1.0f, 1.0f};
+    bool has_sub = false, has_randsub = false, has_aspect = false;
     for (int i = 2; i < argc; ++i) {
         const std::string a = argv[i];
         auto next = [&](const char* what) -> std::string {
@@ -155,6 +203,22 @@ int main(int argc, char** argv) {
             reach = next("--reach");
         } else if (a == "--gpu") {
             gpu = std::atoi(next("--gpu").c_str());
+        } else if (a == "--sub" || a == "--origin" || a == "--randsub") {
+            int* dst = a == "--sub" ? sub : (a == "--origin" ? origin : randsub);
+            int v[3];
+            if (!parse3(next(a.c_str()), v)) {
+                std::cerr << "invalid value for " << a << std::endl;
+                return 1;
+            }
+            std::copy(v, v + 3, dst);
+            has_sub |= (a == "--sub");
+            has_randsub |= (a == "--randsub");
+        } else if (a == "--aspect") {
+            if (!parse3(next("--aspect"), aspect)) {
+                std::cerr << "invalid value for --aspect" << std::endl;
+                return 1;
+            }
+            has_aspect = true;
         } else if (!a.empty() && a[0] != '-' && input.empty()) {
             input = a;
         } else {
@@ -166,16 +230,91 @@ int main(int argc, char** argv) {
         std::cerr << "Invalid input" << std::endl;
         return 1;
     }
-    int nx, ny, nz;
-    if (fast_load_bin_dims(input.c_str(), &nx, &ny, &nz) != FAST_OK) {
-        std::cerr << "Failed to load: " << input << std::endl;
+    // load (loadVolume, VolumeIOModule.h:41-131)
+    int dim[3] = {0, 0, 0};
+    std::vector<uint8_t> mask;
+    if (is_directory(input)) {
+        if (fast_tiff_folder_dims(input.c_str(), &dim[0], &dim[1], &dim[2]) != FAST_OK) {
+            std::cerr << "Failed to load: " << input << std::endl;
+            return 1;
+        }
+        mask.resize((size_t)dim[0] * dim[1] * dim[2]);
+        if (fast_load_tiff_folder(input.c_str(), mask.data(), mask.size()) != FAST_OK) {
+            std::cerr << "Failed to load: " << input << std::endl;
+            return 1;
+        }
+        fast_binarize(mask.data(), mask.size(), 1.0f);
+    } else if (extension(input) == "bin") {
+        if (fast_load_bin_dims(input.c_str(), &dim[0], &dim[1], &dim[2]) != FAST_OK) {
+            std::cerr << "Failed to load: " << input << std::endl;
+            return 1;
+        }
+        mask.resize((size_t)dim[0] * dim[1] * dim[2]);
+        if (fast_load_bin(input.c_str(), mask.data(), mask.size()) != FAST_OK) {
+            std::cerr << "Failed to load: " << input << std::endl;
+            return 1;
+        }
+    } else if (extension(input) == "sph") {
+        // the resolution is the --sub value (VolumeIOModule.h:152-154)
+        if (!has_sub || sub[0] <= 0 || sub[1] <= 0 || sub[2] <= 0) {
+            std::cerr << "Invalid volume: a .sph input needs its resolution as --sub" << std::endl;
+            return 1;
+        }
+        std::copy(sub, sub + 3, dim);
+        mask.resize((size_t)dim[0] * dim[1] * dim[2]);
+        if (fast_rasterize_sph(input.c_str(), dim[0], dim[1], dim[2], mask.data()) != FAST_OK) {
+            std::cerr << "Failed to load: " << input << std::endl;
+            return 1;
+        }
+    } else {
+        std::cerr << "Invalid volume: " << input << std::endl;
         return 1;
     }
-    std::vector<uint8_t> mask((size_t)nx * ny * nz);
-    if (fast_load_bin(input.c_str(), mask.data(), mask.size()) != FAST_OK) {
-        std::cerr << "Failed to load: " << input << std::endl;
-        return 1;
+    // crops (VolumeIOModule.h:159-238)
+    if (has_sub || origin[0] || origin[1] || origin[2]) {
+        int lo[3], hi[3], sz[3];
+        bool ok = true;
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(std::max(origin[a], 0), dim[a]);
+            hi[a] = std::min(std::max(origin[a] + sub[a], 0), dim[a]);
+            sz[a] = hi[a] - lo[a];
+            ok = ok && hi[a] > lo[a];
+        }
+        if (!ok || !crop(mask, dim, lo, sz)) {
+            std::cerr << "Invalid resize options" << std::endl;
+            return 1;
+        }
+    } else if (has_randsub) {
+        std::mt19937 rng((unsigned)std::chrono::system_clock::now().time_since_epoch().count());
+        int lo[3];
+        bool ok = true;
+        for (int a = 0; a < 3; ++a) {
+            const int max_origin = dim[a] - randsub[a];
+            ok = ok && max_origin > 0 && randsub[a] > 0;
+            lo[a] = ok ? (int)(rng() % (unsigned)max_origin) : 0;
+        }
+        if (!ok || !crop(mask, dim, lo, randsub)) {
+            std::cerr << "Invalid resize options" << std::endl;
+            return 1;
+        }
     }
+    if (has_aspect) {
+        int k = -1;
+        for (int a = 0; a < 3 && k < 0; ++a)
+            if (aspect[a] == 1.0f) k = a;
+        if (k < 0) {
+            std::cerr << "Invalid aspect options, one dimension must equal to 1.0" << std::endl;
+            return 1;
+        }
+        int sz[3];
+        const int zero[3] = {0, 0, 0};
+        for (int a = 0; a < 3; ++a) sz[a] = std::min(std::max((int)((float)dim[k] * aspect[a]), 0), dim[a]);
+        if (!crop(mask, dim, zero, sz)) {
+            std::cerr << "Invalid aspect options" << std::endl;
+            return 1;
+        }
+    }
+    const int nx = dim[0], ny = dim[1], nz = dim[2];
     fast_ctx* ctx = nullptr;
     int rc = fast_ctx_create(gpu, 0, &ctx);
     if (rc != FAST_OK) {

@@ -1,4 +1,5 @@
-// Host-side volume utilities of libfasttau: the reference `.bin` format and the seeded
+// Host-side volume utilities of libfasttau: the reference's input formats (`.bin`, TIFF
+// folders, `.sph` sphere lists), binarization and subvolumes, and the seeded
 // synthetic-volume generators used for the benchmark workloads.
 #include "fast_tau.h"
 
@@ -6,6 +7,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <dirent.h>
+#include <fstream>
+#include <string>
+#include <sys/stat.h>
 #include <vector>
 
 namespace {
@@ -75,9 +80,214 @@ size_t fill_quadric(uint8_t* m, int nx, int ny, int nz, const double c[3], const
     return added;
 }
 
+// ---- TIFF (baseline, uncompressed, 8-bit grayscale strips) ---------------------------------
+// The reference reads slices with TinyTIFF (fastlib/src/volume/VolumeIO.cpp:33-77): the first
+// image of each file, raw sample data.  This reader takes the same images and refuses what it
+// cannot read byte for byte as uint8: compression, more than one sample per pixel, sample
+// sizes other than 8 bits, tiles, BigTIFF.
+struct TiffInfo {
+    uint32_t w = 0, h = 0;
+    std::vector<uint64_t> off, cnt;   // strips
+};
+
+bool read_file(const std::string& path, std::vector<uint8_t>& buf) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return false;
+    f.seekg(0, std::ios::end);
+    const std::streamoff n = f.tellg();
+    if (n < 8) return false;
+    buf.resize((size_t)n);
+    f.seekg(0);
+    return (bool)f.read((char*)buf.data(), n);
+}
+
+bool tiff_parse(const std::vector<uint8_t>& b, TiffInfo& t) {
+    const bool be = b[0] == 'M' && b[1] == 'M';
+    if (!be && !(b[0] == 'I' && b[1] == 'I')) return false;
+    auto u16 = [&](size_t o) -> uint32_t {
+        if (o + 2 > b.size()) return 0;
+        return be ? (uint32_t)(b[o] << 8 | b[o + 1]) : (uint32_t)(b[o + 1] << 8 | b[o]);
+    };
+    auto u32 = [&](size_t o) -> uint32_t {
+        if (o + 4 > b.size()) return 0;
+        return be ? ((uint32_t)b[o] << 24 | (uint32_t)b[o + 1] << 16 | (uint32_t)b[o + 2] << 8 | b[o + 3])
+                  : ((uint32_t)b[o + 3] << 24 | (uint32_t)b[o + 2] << 16 | (uint32_t)b[o + 1] << 8 | b[o]);
+    };
+    if (u16(2) != 42) return false;   // 43 = BigTIFF: not read
+    const size_t ifd = u32(4);
+    const uint32_t ne = u16(ifd);
+    if (ifd + 2 + 12ull * ne > b.size()) return false;
+    uint32_t bps = 8, spp = 1, comp = 1, planar = 1, fmt = 1, rps = 0xFFFFFFFFu;
+    bool tiled = false;
+    // values of an entry: SHORT (3) or LONG (4), inline when they fit in 4 bytes
+    auto values = [&](size_t e, std::vector<uint64_t>& out) -> bool {
+        const uint32_t type = u16(e + 2), count = u32(e + 4);
+        const uint32_t sz = type == 3 ? 2 : (type == 4 ? 4 : 0);
+        if (!sz || count == 0 || count > (1u << 28)) return false;
+        const size_t base = (size_t)sz * count <= 4 ? e + 8 : u32(e + 8);
+        if (base + (size_t)sz * count > b.size()) return false;
+        out.resize(count);
+        for (uint32_t i = 0; i < count; ++i) out[i] = sz == 2 ? u16(base + 2 * i) : u32(base + 4 * i);
+        return true;
+    };
+    for (uint32_t k = 0; k < ne; ++k) {
+        const size_t e = ifd + 2 + 12ull * k;
+        const uint32_t tag = u16(e);
+        std::vector<uint64_t> v;
+        switch (tag) {
+            case 256: if (!values(e, v)) return false; t.w = (uint32_t)v[0]; break;
+            case 257: if (!values(e, v)) return false; t.h = (uint32_t)v[0]; break;
+            case 258: if (!values(e, v)) return false; bps = (uint32_t)v[0]; break;
+            case 259: if (!values(e, v)) return false; comp = (uint32_t)v[0]; break;
+            case 273: if (!values(e, t.off)) return false; break;
+            case 277: if (!values(e, v)) return false; spp = (uint32_t)v[0]; break;
+            case 278: if (!values(e, v)) return false; rps = (uint32_t)v[0]; break;
+            case 279: if (!values(e, t.cnt)) return false; break;
+            case 284: if (!values(e, v)) return false; planar = (uint32_t)v[0]; break;
+            case 339: if (!values(e, v)) return false; fmt = (uint32_t)v[0]; break;
+            case 322: case 323: case 324: case 325: tiled = true; break;
+            default: break;
+        }
+    }
+    (void)planar;
+    if (tiled || comp != 1 || spp != 1 || bps != 8 || fmt != 1 || t.w == 0 || t.h == 0) return false;
+    if (t.off.empty() || t.off.size() != t.cnt.size()) return false;
+    (void)rps;
+    return true;
+}
+
+// One slice: the strips in order, w * h bytes.
+bool tiff_read(const std::string& path, uint32_t* w, uint32_t* h, uint8_t* dst, size_t cap) {
+    std::vector<uint8_t> b;
+    TiffInfo t;
+    if (!read_file(path, b) || !tiff_parse(b, t)) return false;
+    *w = t.w;
+    *h = t.h;
+    if (!dst) return true;
+    const size_t need = (size_t)t.w * t.h;
+    if (cap < need) return false;
+    size_t pos = 0;
+    for (size_t i = 0; i < t.off.size() && pos < need; ++i) {
+        const size_t n = std::min<size_t>((size_t)t.cnt[i], need - pos);
+        if (t.off[i] + n > b.size()) return false;
+        std::memcpy(dst + pos, b.data() + t.off[i], n);
+        pos += n;
+    }
+    return pos == need;
+}
+
+bool is_dir(const char* p) {
+    struct stat st;
+    return p && stat(p, &st) == 0 && S_ISDIR(st.st_mode);
+}
+
+// listDir + checkExtension (VolumeIO.cpp:79-111,307-310): names sorted, extension after the
+// last '.' equal to "tiff" or "tif" (case-sensitive), directories skipped.
+std::vector<std::string> tiff_slices(const char* folder) {
+    std::vector<std::string> out;
+    DIR* d = opendir(folder);
+    if (!d) return out;
+    while (dirent* e = readdir(d)) {
+        const std::string name = e->d_name;
+        const size_t dot = name.find_last_of('.');
+        const std::string ext = dot == std::string::npos ? name : name.substr(dot + 1);
+        if (ext != "tiff" && ext != "tif") continue;
+        const std::string full = std::string(folder) + "/" + name;
+        if (is_dir(full.c_str())) continue;
+        out.push_back(name);
+    }
+    closedir(d);
+    std::sort(out.begin(), out.end());
+    return out;
+}
+
 }  // namespace
 
 extern "C" {
+
+int fast_tiff_folder_dims(const char* folder, int* nx, int* ny, int* nz) {
+    if (!folder || !nx || !ny || !nz) return FAST_E_BADARG;
+    if (!is_dir(folder)) return FAST_E_IO;
+    const std::vector<std::string> sl = tiff_slices(folder);
+    if (sl.empty()) return FAST_E_IO;
+    uint32_t w = 0, h = 0;
+    if (!tiff_read(std::string(folder) + "/" + sl[0], &w, &h, nullptr, 0)) return FAST_E_IO;
+    if ((uint64_t)w * h * sl.size() > ((uint64_t)1 << 40) || w > 0x7FFFFFFF || h > 0x7FFFFFFF) return FAST_E_IO;
+    *nx = (int)w;
+    *ny = (int)h;
+    *nz = (int)sl.size();
+    return FAST_OK;
+}
+
+int fast_load_tiff_folder(const char* folder, uint8_t* mask, size_t capacity) {
+    int nx, ny, nz;
+    int rc = fast_tiff_folder_dims(folder, &nx, &ny, &nz);
+    if (rc) return rc;
+    const size_t plane = (size_t)nx * ny;
+    if (!mask || capacity < plane * nz) return FAST_E_BADARG;
+    const std::vector<std::string> sl = tiff_slices(folder);
+    for (size_t k = 0; k < sl.size(); ++k) {
+        uint32_t w, h;
+        if (!tiff_read(std::string(folder) + "/" + sl[k], &w, &h, mask + k * plane, plane)) return FAST_E_IO;
+        if ((int)w != nx || (int)h != ny) return FAST_E_IO;   // slices of another size
+    }
+    return FAST_OK;
+}
+
+void fast_binarize(uint8_t* mask, size_t n, float threshold) {
+    if (!mask) return;
+    const uint8_t t = (uint8_t)(threshold * 255);   // uchar(threshold * 255), Volume.cu:180
+    for (size_t i = 0; i < n; ++i) mask[i] = mask[i] < t ? 0 : 255;
+}
+
+int fast_rasterize_sph(const char* path, int nx, int ny, int nz, uint8_t* mask) {
+    if (!path || !mask || nx <= 0 || ny <= 0 || nz <= 0) return FAST_E_BADARG;
+    std::ifstream f(path);
+    if (!f) return FAST_E_IO;
+    long count = 0;
+    if (!(f >> count) || count < 0) return FAST_E_IO;
+    struct Sph {
+        float x, y, z, r, r2;
+    };
+    std::vector<Sph> sp((size_t)count);
+    for (auto& s : sp) {
+        if (!(f >> s.x >> s.y >> s.z >> s.r)) return FAST_E_IO;
+        s.r2 = s.r * s.r;
+    }
+    std::memset(mask, 0, (size_t)nx * ny * nz);
+    const int res[3] = {nx, ny, nz};
+    for (const Sph& s : sp) {
+        const float c[3] = {s.x, s.y, s.z};
+        int lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            // normToDiscrete<true>: int((dim - 1) * p), then clamped to [0, dim - 1]
+            lo[a] = std::min(std::max((int)((float)(res[a] - 1) * (c[a] - s.r)), 0), res[a] - 1);
+            hi[a] = std::min(std::max((int)((float)(res[a] - 1) * (c[a] + s.r)), 0), res[a] - 1);
+        }
+        for (int z = lo[2]; z <= hi[2]; ++z)
+            for (int y = lo[1]; y <= hi[1]; ++y)
+                for (int x = lo[0]; x <= hi[0]; ++x) {
+                    // discreteToNorm: float(i) / dim; glm::length2 summed x, y, z
+                    const float dx = (float)x / (float)nx - s.x;
+                    const float dy = (float)y / (float)ny - s.y;
+                    const float dz = (float)z / (float)nz - s.z;
+                    const float d2 = dx * dx + dy * dy + dz * dz;
+                    if (d2 < s.r2) mask[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)] = 255;
+                }
+    }
+    return FAST_OK;
+}
+
+int fast_crop(const uint8_t* src, int nx, int ny, int nz, int ox, int oy, int oz, int sx, int sy, int sz, uint8_t* dst) {
+    if (!src || !dst || nx <= 0 || ny <= 0 || nz <= 0 || sx <= 0 || sy <= 0 || sz <= 0 || ox < 0 || oy < 0 || oz < 0 ||
+        ox + sx > nx || oy + sy > ny || oz + sz > nz)
+        return FAST_E_BADARG;
+    for (int z = 0; z < sz; ++z)
+        for (int y = 0; y < sy; ++y)
+            std::memcpy(dst + ((size_t)z * sy + y) * sx, src + (((size_t)(z + oz) * ny + (y + oy)) * nx + ox), (size_t)sx);
+    return FAST_OK;
+}
+
 
 int fast_load_bin_dims(const char* path, int* nx, int* ny, int* nz) {
     FILE* f = std::fopen(path, "rb");

@@ -382,6 +382,42 @@ def save_bin(path: str, mask: np.ndarray) -> None:
     check(lib().fast_save_bin(str(path).encode(), _u8(m), nx, ny, nz), "fast_save_bin")
 
 
+def loadTiffFolder(folder: str) -> np.ndarray:
+    """loadTiffFolder (VolumeIO.cpp:195-253): the .tif/.tiff slices of `folder` in sorted name
+    order as a [z, y, x] uint8 volume (raw values; the CLI binarizes them)."""
+    nx, ny, nz = C.c_int(), C.c_int(), C.c_int()
+    check(lib().fast_tiff_folder_dims(str(folder).encode(), C.byref(nx), C.byref(ny), C.byref(nz)),
+          "fast_tiff_folder_dims")
+    m = np.empty((nz.value, ny.value, nx.value), dtype=np.uint8)
+    check(lib().fast_load_tiff_folder(str(folder).encode(), _u8(m), m.size), "fast_load_tiff_folder")
+    return m
+
+
+def binarize(mask: np.ndarray, threshold: float = 1.0) -> np.ndarray:
+    """Volume::binarize on uint8 (Volume.cu:152-182): v < uchar(threshold * 255) -> 0, else 255."""
+    m = np.ascontiguousarray(mask, dtype=np.uint8).copy()
+    lib().fast_binarize(_u8(m), m.size, threshold)
+    return m
+
+
+def rasterizeSpheres(path: str, shape) -> np.ndarray:
+    """A `.sph` sphere list (VolumeGenerator.cpp:273-286) rasterized at `shape` = (nx, ny, nz)
+    (rasterizeSpheres, VolumeGenerator.cpp:210-251), as a [z, y, x] volume."""
+    nx, ny, nz = (shape, shape, shape) if np.isscalar(shape) else shape
+    m = np.empty((nz, ny, nx), dtype=np.uint8)
+    check(lib().fast_rasterize_sph(str(path).encode(), nx, ny, nz, _u8(m)), "fast_rasterize_sph")
+    return m
+
+
+def getSubvolume(mask: np.ndarray, origin, size) -> np.ndarray:
+    """Volume::getSubvolume (Volume.cpp:129-160): the box [origin, origin + size), (x, y, z)."""
+    m, nx, ny, nz = _mask_xyz(mask)
+    sx, sy, sz = size
+    out = np.empty((sz, sy, sx), dtype=np.uint8)
+    check(lib().fast_crop(_u8(m), nx, ny, nz, origin[0], origin[1], origin[2], sx, sy, sz, _u8(out)), "fast_crop")
+    return out
+
+
 def generate_spheres(shape, radius: float, solid_fraction: float, seed: int) -> np.ndarray:
     """[z, y, x] volume of overlapping spheres (255 = solid) filled to `solid_fraction`."""
     nx, ny, nz = (shape, shape, shape) if np.isscalar(shape) else shape

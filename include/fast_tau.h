@@ -205,6 +205,28 @@ void fast_server_destroy(fast_server* srv);
 int fast_load_bin_dims(const char* path, int* nx, int* ny, int* nz);
 int fast_load_bin(const char* path, uint8_t* mask, size_t capacity);
 int fast_save_bin(const char* path, const uint8_t* mask, int nx, int ny, int nz);
+/* TIFF folder (loadTiffFolder, fastlib/src/volume/VolumeIO.cpp:195-253): every file whose name
+ * ends in ".tiff" or ".tif" (case-sensitive, VolumeIO.cpp:307-310), in sorted name order
+ * (listDir, VolumeIO.cpp:79-111), is one z slice; nx, ny come from the first.  Baseline
+ * uncompressed 8-bit grayscale strips in either byte order, the first image of each file.
+ * FAST_E_IO for anything else (compression, other sample sizes, tiles, BigTIFF) and for slices
+ * of another size; the reference's reader copies other sample sizes into the uint8 slice
+ * unchecked. */
+int fast_tiff_folder_dims(const char* folder, int* nx, int* ny, int* nz);
+int fast_load_tiff_folder(const char* folder, uint8_t* mask, size_t capacity);
+/* Volume::binarize on uint8 (fastlib/src/cuda/Volume.cu:152-182): t = uchar(threshold * 255),
+ * v = v < t ? 0 : 255.  The CLI binarizes TIFF input with threshold 1 (VolumeIOModule.h:55-60). */
+void fast_binarize(uint8_t* mask, size_t n, float threshold);
+/* `.sph` input (VolumeIOModule.h:86-96): loadSpheres (VolumeGenerator.cpp:273-286: a count,
+ * then x y z r per sphere in normalized coordinates) rasterized at nx x ny x nz by
+ * rasterizeSpheres (VolumeGenerator.cpp:210-251): 255 where float(i)/dim lies strictly inside a
+ * sphere (float arithmetic), scanning int((dim-1)(c -+ r)) clamped to the box. */
+int fast_rasterize_sph(const char* path, int nx, int ny, int nz, uint8_t* mask);
+/* Subvolume (Volume::getSubvolume, fastlib/src/volume/Volume.cpp:129-160, used by the CLI's
+ * --sub/--origin/--randsub/--aspect crops, VolumeIOModule.h:159-245): dst (sx*sy*sz) = the box
+ * [o, o + s) of src. */
+int fast_crop(const uint8_t* src, int nx, int ny, int nz, int ox, int oy, int oz, int sx, int sy, int sz,
+              uint8_t* dst);
 
 /* Seeded generators (255 = solid, 0 = pore): bodies are added with centres uniform in the
  * box, overlap allowed, until the solid fraction reaches `solid_fraction`; a voxel is solid

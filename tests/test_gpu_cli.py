@@ -34,3 +34,64 @@ def test_cli_config1_tau_and_reachable_mask(tmp_path):
     got = F.load_bin(str(reach))
     assert got.shape == m.shape
     assert np.array_equal(got, mask_ref)
+
+
+def _run(args, tmp_path, name):
+    out = tmp_path / f"{name}.csv"
+    cmd = [str(EXE), "tau", *args, "-d", "x", "--solver", "CG", "--tol", "8", "-o", str(out)]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, (args, r.stderr)
+    rows = list(csv.DictReader(open(out)))
+    assert len(rows) == 1
+    return rows[0]
+
+
+def test_cli_tiff_folder_equals_bin(tmp_path):
+    """A folder of TIFF slices is binarized with threshold 1 (values below 255 are pore) and
+    solved like the same volume given as .bin; both equal the oracle."""
+    from PIL import Image
+    m = F.generate_spheres((40, 36, 32), 4.0, 0.5, 3)
+    raw = m.copy()
+    raw[(m == 0) & (np.arange(m.size).reshape(m.shape) % 7 == 0)] = 100   # binarizes to pore
+    d = tmp_path / "slices"
+    d.mkdir()
+    for k, sl in enumerate(raw):
+        Image.fromarray(sl, mode="L").save(d / f"slice_{k:03d}.tif")
+    F.save_bin(str(tmp_path / "v.bin"), m)
+    a = _run([str(d)], tmp_path, "tiff")
+    b = _run([str(tmp_path / "v.bin")], tmp_path, "bin")
+    assert (a["dimx"], a["dimy"], a["dimz"]) == ("40", "36", "32")
+    assert a["tau"] == b["tau"] and a["tauIter"] == b["tauIter"] and a["porosity"] == b["porosity"]
+    ref, _ = O.tau(m, 0, solver=1, tol=1e-8)
+    assert float(a["tau"]) == pytest.approx(ref.tau, rel=5e-6)
+    assert int(a["tauIter"]) == ref.iterations
+
+
+def test_cli_sph_and_crops(tmp_path):
+    """.sph rasterized at the --sub resolution; --sub/--origin and --aspect crops equal the
+    same subvolumes written as .bin; --randsub gives the requested size."""
+    sph = tmp_path / "s.sph"
+    sph.write_text("3\n0.5 0.5 0.5 0.2\n0.2 0.8 0.3 0.15\n0.8 0.2 0.7 0.18\n")
+    a = _run([str(sph), "--sub", "40,36,32"], tmp_path, "sph")
+    m = F.rasterizeSpheres(str(sph), (40, 36, 32))
+    F.save_bin(str(tmp_path / "sph.bin"), m)
+    b = _run([str(tmp_path / "sph.bin")], tmp_path, "sphbin")
+    assert a["tau"] == b["tau"] and a["tauIter"] == b["tauIter"]
+
+    v = F.generate_spheres(48, 4.0, 0.45, 5)
+    F.save_bin(str(tmp_path / "v.bin"), v)
+    c = _run([str(tmp_path / "v.bin"), "--sub", "30,28,26", "--origin", "5,6,7"], tmp_path, "crop")
+    F.save_bin(str(tmp_path / "c.bin"), v[7:33, 6:34, 5:35])
+    c2 = _run([str(tmp_path / "c.bin")], tmp_path, "cropbin")
+    assert (c["dimx"], c["dimy"], c["dimz"]) == ("30", "28", "26")
+    assert c["tau"] == c2["tau"] and c["tauIter"] == c2["tauIter"]
+
+    e = _run([str(tmp_path / "v.bin"), "--aspect", "1,0.5,0.75"], tmp_path, "aspect")
+    F.save_bin(str(tmp_path / "e.bin"), v[:36, :24, :48])
+    e2 = _run([str(tmp_path / "e.bin")], tmp_path, "aspectbin")
+    assert (e["dimx"], e["dimy"], e["dimz"]) == ("48", "24", "36")
+    assert e["tau"] == e2["tau"]
+
+    r = _run([str(tmp_path / "v.bin"), "--randsub", "20"], tmp_path, "randsub")
+    assert (r["dimx"], r["dimy"], r["dimz"]) == ("20", "20", "20") and float(r["tau"]) > 0
