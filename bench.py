@@ -71,8 +71,17 @@ def cpu_baseline(mask: np.ndarray, gpu_iters: float, threads: int, sample_iters:
     t_run = time.perf_counter() - t
     per_iter = max(t_run - t_setup, 1e-9) / sample_iters
     solve_s = t_setup + per_iter * gpu_iters
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {
         "value": 1.0 / solve_s, "unit": "solves/s", "cores": threads, "kind": "port",
+        "cpu_model": model, "nproc": os.cpu_count(),
         "sample": (f"256^3 volume, X_POS, CG FP64 (oracle restating the reference --cpu path): assembly "
                    f"{t_setup:.2f} s + {sample_iters} iterations at {per_iter * 1e3:.0f} ms each, extrapolated to "
                    f"{gpu_iters:.0f} iterations = {solve_s:.1f} s per solve"),

@@ -1670,8 +1670,15 @@ struct RbTabs {
     FacePair<double> ftab[8];
 };
 
+// PART: 0 diagonals only, 1 inverses only, 2 both (a pass copies only what it reads; the
+// other half of the LDS table stays unset and unread).
+template <int PART = 2>
 __device__ __forceinline__ void rb_load_tables(const RbLaunch& L, int v, const Coef<double>& c, RbTabs& t) {
-    load_tables<double, kRbTX * kRbTY>(L.tab + (long long)v * 2 * kTabEntries, t.tab, t.ftab, c);
+    const double* src = L.tab + (long long)v * 2 * kTabEntries;
+    constexpr int lo = PART == 1 ? kTabEntries : 0, hi = PART == 0 ? kTabEntries : 2 * kTabEntries;
+    for (int i = lo + (int)threadIdx.x; i < hi; i += kRbTX * kRbTY) t.tab[i] = src[i];
+    if (threadIdx.x < 8) t.ftab[threadIdx.x] = face_pair(c, threadIdx.x);
+    __syncthreads();
 }
 
 // Loads of the six neighbours of the other colour, z-, y-, x-, x+, y+, z+, of the voxel
@@ -1733,7 +1740,7 @@ __global__ __launch_bounds__(kRbTX* kRbTY) void k_rb_it(RbLaunch L) {
     if (vs->status != ST_RUNNING) return;
     __shared__ RbTabs tabs;
     const Coef<double> c = L.coef[v];
-    rb_load_tables(L, v, c, tabs);
+    rb_load_tables<KIND == RB_Q ? 0 : 1>(L, v, c, tabs);   // Q reads D; P, Y and U read D^-1
     const RbPos ps = rb_pos(d, tile - d.tile_base);
     constexpr int kColour = (KIND == RB_Y) ? 1 : 0;
     const double alpha = vs->alpha, beta = vs->beta;
@@ -1840,7 +1847,7 @@ __global__ __launch_bounds__(kRbTX* kRbTY) void k_rb_q(RbLaunch L) {
     __shared__ RbTabs tabs;
     __shared__ U ring[4 * PL];
     const Coef<double> c = L.coef[v];
-    rb_load_tables(L, v, c, tabs);
+    rb_load_tables<0>(L, v, c, tabs);   // Q reads D only
     const int lt = tile - d.tile_base;
     const int k0 = (lt % d.tiles_x) * kRbTX;
     const int y0 = ((lt / d.tiles_x) % d.tiles_y) * kRbTY;
