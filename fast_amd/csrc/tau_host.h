@@ -48,6 +48,7 @@ struct fast_ctx {
     fth::Shape forced{0, 0, 0};                // tile shape forced by fast_ctx_set_tile_shape
     int chunk = 16;                            // Krylov iterations per polling round
     bool use_graphs = true;                    // replay polling rounds from a captured HIP graph
+    size_t group_bytes = (size_t)320 << 20;    // red-black batches: state bytes per sub-batch (0 = one launch)
     double mixed_restart_cos = 1e-6;           // mixed form: restart when r0 and r are this close to orthogonal
     std::unordered_map<fast_volume*, int> volumes;   // live volumes, released with the context
     std::mutex mu;

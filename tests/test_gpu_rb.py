@@ -138,3 +138,21 @@ def test_staged_q_pass_bit_identical(ctx, monkeypatch):
             assert a.tau == b.tau and a.iterations == b.iterations, (m.shape, d, a.tau, b.tau)
             assert np.array_equal(ca, cb), (m.shape, d)
         vol.close()
+
+
+def test_large_volume_batch_runs_in_cache_sized_groups(monkeypatch):
+    """A batch of volumes that are each large for the last-level cache runs as consecutive
+    sub-batches (here the group size is lowered to 8 MB so that 64^3 volumes count as large);
+    every result stays bit-identical to its solo solve."""
+    monkeypatch.setenv("FAST_TAU_GROUP_MB", "8")
+    c = T.Context(0)
+    try:
+        vols = [c.volume(F.generate_spheres(64, 4.0 + i, 0.5, 60 + i)) for i in range(3)]
+        ps = [T.make_params(d, CG, "mixed", 1e-8, 10000) for d in (0, 2, 5)]
+        batch = T.solve_batch(c, vols, ps)
+        for v, p, b in zip(vols, ps, batch):
+            solo, _ = v.solve(p)
+            assert b.status == 0 and b.tau == solo.tau and b.iterations == solo.iterations
+            v.close()
+    finally:
+        c.close()
