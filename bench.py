@@ -192,6 +192,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-iters", type=int, default=25)
     ap.add_argument("--serve", type=int, default=120, help="requests per offered load (0: skip serving)")
+    ap.add_argument("--no-dram-check", dest="dram_check", action="store_false",
+                    help="skip the red-black pass timings on the 512^3 needle volume")
     ap.add_argument("--profile", choices=["solves", "passes"],
                     help="short program for rocprofv3 instead of the benchmark: 'solves' runs the X_POS solves "
                          "(FP64 BiCGSTAB, FP64 CG, mixed CG red-black and full grid) with direct launches; 'passes' "
@@ -283,6 +285,25 @@ def main() -> int:
                   "traffic": measured_traffic(k.split(" ")[0], n == 256)}
               for k, v in kernels.items()}
     iter_bytes = sum(v[1] for v in kernels.values())
+    # the red-black state of one 256^3 volume (~260 MB) is about the size of the 256 MB
+    # Infinity Cache, so part of the 256^3 traffic may be served from it; the same passes on
+    # the 512^3 needle volume (~3 GB of state) are DRAM-bound
+    dram = None
+    if args.dram_check:
+        log("[bench] red-black passes on the 512^3 needle volume")
+        vol.close()
+        big = ctx.volume(W.needles_512(0))
+        lib().fast_debug_time_rb_passes(big.handle, 0, 10, rb)
+        big.close()
+        vol = ctx.volume(mask)
+        nb = 512 ** 3
+        nr5 = nb // 2
+        b5 = {"k_rb_it<0> (P)": 13 * nr5, "k_rb_it<1> (Y)": 13 * nr5, "k_rb_q (Q)": 17 * nr5, "k_rb_it<3> (U)": 25 * nr5}
+        dram = {"volume": "512^3 needles a=12 r=0.1 solid 0.55 seed 0, X_POS",
+                "kernels": {k: {"avg_launch_ms": rb[i], "bytes_per_launch": b,
+                                "achieved_GBs": b / (rb[i] * 1e-3) / 1e9, "frac": b / (rb[i] * 1e-3) / 8e12}
+                            for i, (k, b) in enumerate(b5.items())},
+                "iteration_ms": rb[4], "iteration_GBs": sum(b5.values()) / (rb[4] * 1e-3) / 1e9}
 
     cpu = None
     if not args.no_cpu:
@@ -342,6 +363,7 @@ def main() -> int:
         "kernels_mixed_cg": others,
         "mixed_cg_iteration": {"ms": rb_pass[4], "algorithmic_bytes": iter_bytes,
                                "achieved_GBs": iter_bytes / (rb_pass[4] * 1e-3) / 1e9},
+        "mixed_cg_passes_512_dram": dram,
         "pass_ms": {"mixed_cg_red_black": rb_pass, "mixed_cg_full_grid": mixed_full_pass, "fp64_cg": fp64_cg_pass,
                     "fp64_bicgstab": fp64_bicg_pass},
         "cpu_baseline": cpu,
