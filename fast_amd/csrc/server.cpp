@@ -558,7 +558,7 @@ void worker_main(fast_server* s) {
         hipStream_t st = ctx->stream;
         hipError_t e = hipSuccess;
         for (int i = 0; i < m && e == hipSuccess; ++i) {
-            const bool upd = (i + 1) % kMixedUpdateEvery == 0 || i + 1 == m;
+            const bool upd = (i + 1) % ctx->update_every == 0 || i + 1 == m;
             if (s->rb) {
                 e = ft::rb_iter(s->RL, st);
                 if (e == hipSuccess && upd) e = ft::rb_update(s->RL, st);

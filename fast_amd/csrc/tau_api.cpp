@@ -269,6 +269,7 @@ int fast_ctx_create(int device, size_t hbm_budget, fast_ctx** out) {
     if (const char* e = std::getenv("FAST_TAU_MIXED_RESTART")) ctx->mixed_restart_cos = std::atof(e);
     if (const char* e = std::getenv("FAST_TAU_GRAPHS")) ctx->use_graphs = std::atoi(e) != 0;
     if (const char* e = std::getenv("FAST_TAU_GROUP_MB")) ctx->group_bytes = (size_t)std::atol(e) << 20;
+    if (const char* e = std::getenv("FAST_TAU_UPDATE_EVERY")) ctx->update_every = std::max(1, std::atoi(e));
     *out = ctx;
     return FAST_OK;
 }
@@ -770,7 +771,7 @@ static int solve_mixed(fast_ctx* ctx, std::vector<Job*>& jobs, Shape shape, int 
             e2 = (solver == FAST_BICGSTAB) ? Passes<float, double>::bicg_iter(B.L, s) : Passes<float, double>::cg_iter(B.L, s);
             // A volume that asked for an update skips the iterations until the update step
             // runs, so running it every few iterations changes nothing but the idle launches.
-            const bool upd = (i + 1) % kMixedUpdateEvery == 0 || i + 1 == chunk;
+            const bool upd = (i + 1) % ctx->update_every == 0 || i + 1 == chunk;
             if (e2 == hipSuccess && upd) e2 = ft::mixed_update(B.L, s);
             if (e2 == hipSuccess && upd) e2 = ft::mixed_residual(B.L, B.coef64, s);
         }
@@ -1023,7 +1024,7 @@ static int solve_mixed_rb(fast_ctx* ctx, std::vector<Job*>& jobs) {
         hipError_t e2 = hipSuccess;
         for (int i = 0; i < chunk && e2 == hipSuccess; ++i) {
             e2 = ft::rb_iter(B.L, s);
-            const bool upd = (i + 1) % kMixedUpdateEvery == 0 || i + 1 == chunk;
+            const bool upd = (i + 1) % ctx->update_every == 0 || i + 1 == chunk;
             if (e2 == hipSuccess && upd) e2 = ft::rb_update(B.L, s);
         }
         return e2;

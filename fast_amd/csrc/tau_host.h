@@ -49,6 +49,7 @@ struct fast_ctx {
     int chunk = 16;                            // Krylov iterations per polling round
     bool use_graphs = true;                    // replay polling rounds from a captured HIP graph
     size_t group_bytes = (size_t)320 << 20;    // red-black batches: state bytes per sub-batch (0 = one launch)
+    int update_every = fth::kMixedUpdateEvery; // mixed form: iterations between reliable-update steps
     double mixed_restart_cos = 1e-6;           // mixed form: restart when r0 and r are this close to orthogonal
     std::unordered_map<fast_volume*, int> volumes;   // live volumes, released with the context
     std::mutex mu;

@@ -101,7 +101,9 @@ typedef struct fast_ctx fast_ctx;
  * a captured HIP graph (same kernels, same results); FAST_TAU_TUNING names a tile-shape table;
  * FAST_TAU_MIXED_RESTART sets the mixed form's shadow-residual reset threshold;
  * FAST_TAU_GROUP_MB (default 320) is the red-black state size of the consecutive sub-batches a
- * batch of large volumes runs in (0 = always one launch; results are the same either way). */
+ * batch of large volumes runs in (0 = always one launch; results are the same either way);
+ * FAST_TAU_UPDATE_EVERY (default 4) is the number of mixed-form iterations between launches of
+ * the reliable-update step (results are the same for every value). */
 int fast_ctx_create(int device, size_t hbm_budget, fast_ctx** out);
 void fast_ctx_destroy(fast_ctx* ctx);
 /* Last error message of this context (never NULL). */
