@@ -249,10 +249,12 @@ int fast_rasterize_sph(const char* path, int nx, int ny, int nz, uint8_t* mask) 
     struct Sph {
         float x, y, z, r, r2;
     };
-    std::vector<Sph> sp((size_t)count);
-    for (auto& s : sp) {
+    std::vector<Sph> sp;   // grown as spheres are read: a bad count cannot reserve memory
+    for (long i = 0; i < count; ++i) {
+        Sph s;
         if (!(f >> s.x >> s.y >> s.z >> s.r)) return FAST_E_IO;
         s.r2 = s.r * s.r;
+        sp.push_back(s);
     }
     std::memset(mask, 0, (size_t)nx * ny * nz);
     const int res[3] = {nx, ny, nz};
