@@ -20,6 +20,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <new>
 #include <string>
 #include <unordered_map>
 #include <tuple>
@@ -1151,7 +1152,12 @@ int fast_tau_solve_volume(fast_volume* vol, const fast_tau_params* params, fast_
     jobs[0].prm = *params;
     jobs[0].res = result;
     jobs[0].conc = conc_out;
-    return solve_jobs(ctx, jobs);
+    try {
+        return solve_jobs(ctx, jobs);
+    } catch (const std::bad_alloc&) {   // host staging buffers; device state is released by its owners
+        set_error(ctx, "host allocation failed");
+        return FAST_E_OOM;
+    }
 }
 
 int fast_tau_solve(fast_ctx* ctx, const uint8_t* mask, int nx, int ny, int nz, const fast_tau_params* params,
@@ -1181,7 +1187,12 @@ int fast_tau_solve_batch(fast_ctx* ctx, int count, fast_volume* const* vols, con
         jobs[i].res = &results[i];
         jobs[i].conc = nullptr;
     }
-    return solve_jobs(ctx, jobs);
+    try {
+        return solve_jobs(ctx, jobs);
+    } catch (const std::bad_alloc&) {
+        set_error(ctx, "host allocation failed");
+        return FAST_E_OOM;
+    }
 }
 
 }  // extern "C"

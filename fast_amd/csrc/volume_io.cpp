@@ -9,6 +9,7 @@
 #include <cstring>
 #include <dirent.h>
 #include <fstream>
+#include <new>
 #include <string>
 #include <sys/stat.h>
 #include <vector>
@@ -205,7 +206,7 @@ std::vector<std::string> tiff_slices(const char* folder) {
 
 extern "C" {
 
-int fast_tiff_folder_dims(const char* folder, int* nx, int* ny, int* nz) {
+static int tiff_folder_dims(const char* folder, int* nx, int* ny, int* nz) {
     if (!folder || !nx || !ny || !nz) return FAST_E_BADARG;
     if (!is_dir(folder)) return FAST_E_IO;
     const std::vector<std::string> sl = tiff_slices(folder);
@@ -219,9 +220,9 @@ int fast_tiff_folder_dims(const char* folder, int* nx, int* ny, int* nz) {
     return FAST_OK;
 }
 
-int fast_load_tiff_folder(const char* folder, uint8_t* mask, size_t capacity) {
+static int load_tiff_folder(const char* folder, uint8_t* mask, size_t capacity) {
     int nx, ny, nz;
-    int rc = fast_tiff_folder_dims(folder, &nx, &ny, &nz);
+    int rc = tiff_folder_dims(folder, &nx, &ny, &nz);
     if (rc) return rc;
     const size_t plane = (size_t)nx * ny;
     if (!mask || capacity < plane * nz) return FAST_E_BADARG;
@@ -234,13 +235,41 @@ int fast_load_tiff_folder(const char* folder, uint8_t* mask, size_t capacity) {
     return FAST_OK;
 }
 
+static int rasterize_sph(const char* path, int nx, int ny, int nz, uint8_t* mask);
+
+// The readers allocate host buffers: a failed allocation is FAST_E_OOM, never an exception
+// through the C ABI.
+int fast_tiff_folder_dims(const char* folder, int* nx, int* ny, int* nz) {
+    try {
+        return tiff_folder_dims(folder, nx, ny, nz);
+    } catch (const std::bad_alloc&) {
+        return FAST_E_OOM;
+    }
+}
+
+int fast_load_tiff_folder(const char* folder, uint8_t* mask, size_t capacity) {
+    try {
+        return load_tiff_folder(folder, mask, capacity);
+    } catch (const std::bad_alloc&) {
+        return FAST_E_OOM;
+    }
+}
+
+int fast_rasterize_sph(const char* path, int nx, int ny, int nz, uint8_t* mask) {
+    try {
+        return rasterize_sph(path, nx, ny, nz, mask);
+    } catch (const std::bad_alloc&) {
+        return FAST_E_OOM;
+    }
+}
+
 void fast_binarize(uint8_t* mask, size_t n, float threshold) {
     if (!mask) return;
     const uint8_t t = (uint8_t)(threshold * 255);   // uchar(threshold * 255), Volume.cu:180
     for (size_t i = 0; i < n; ++i) mask[i] = mask[i] < t ? 0 : 255;
 }
 
-int fast_rasterize_sph(const char* path, int nx, int ny, int nz, uint8_t* mask) {
+static int rasterize_sph(const char* path, int nx, int ny, int nz, uint8_t* mask) {
     if (!path || !mask || nx <= 0 || ny <= 0 || nz <= 0) return FAST_E_BADARG;
     std::ifstream f(path);
     if (!f) return FAST_E_IO;
