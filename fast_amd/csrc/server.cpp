@@ -24,6 +24,7 @@
 #include <deque>
 #include <map>
 #include <mutex>
+#include <new>
 #include <thread>
 #include <vector>
 
@@ -650,8 +651,14 @@ int fast_server_submit(fast_server* s, const uint8_t* mask, int nx, int ny, int 
     if (params->dir < 0 || params->dir > 5 || !(params->tolerance > 0) || params->max_iter < 0) return FAST_E_BADARG;
     const size_t n = (size_t)nx * ny * nz;
     if (n > kMaxSolveVoxels) return FAST_E_BADARG;
-    auto* r = new Req();
-    r->mask.assign(mask, mask + n);
+    Req* r = new (std::nothrow) Req();
+    if (!r) return FAST_E_OOM;
+    try {
+        r->mask.assign(mask, mask + n);   // the request keeps its own copy until upload
+    } catch (const std::bad_alloc&) {
+        delete r;
+        return FAST_E_OOM;
+    }
     r->nx = nx;
     r->ny = ny;
     r->nz = nz;
