@@ -640,7 +640,13 @@ int fast_server_create(fast_ctx* ctx, const fast_server_config* cfg, fast_server
         fast_server_destroy(s);
         return FAST_E_HIP;
     }
-    s->worker = std::thread(worker_main, s);
+    s->worker = std::thread([s] {
+        try {
+            worker_main(s);
+        } catch (const std::bad_alloc&) {   // host bookkeeping: fail the requests, keep the process
+            fail_all(s, FAST_E_OOM);
+        }
+    });
     *out = s;
     return FAST_OK;
 }
