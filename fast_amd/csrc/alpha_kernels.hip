@@ -1,0 +1,290 @@
+// Reactive area density: marching-cubes area of the isovalue level set of a volume.
+//
+// Reference (`fast alpha`): getReactiveAreaDensity / getReactiveAreaDensityTensor
+// (fastlib/src/volume/VolumeMeasures.cpp:294-333) -> getVolumeArea
+// (fastlib/src/volume/VolumeSurface.cpp:9-36) -> ___generateTrianglesSurfaceArea
+// (fastlib/src/cuda/VolumeSurface.cu:203-258), then Volume::sum of the per-voxel areas.
+//
+// The reference samples the volume as a linearly filtered, clamped, normalized-coordinate
+// uchar texture (fastlib/src/utility/DataPtr.cpp:565-592) at the corners of every voxel
+// (VolumeSurface.cu:83-103, positions i/N).  A corner sits on the midpoint between texel
+// centres in every axis, so its value is the mean of the 8 voxels around it (indices clamped
+// into the volume), byte/255: (sum of the 8 bytes) / 2040, which for a 0/255 volume is k/8
+// exactly.  Here those corner values are built from voxel sums instead of texture reads:
+// a block keeps, for its 64x4 tile and the plane being marched, the 2x2 in-plane voxel sums
+// of every lattice point (LDS), adds two consecutive planes of them into corner counts, and
+// each thread turns the 8 corner counts of its cube into the cube's triangles and area.
+//
+// Two modes share the march.  MODE 0 reads the volume's bytes (one surface).  MODE 1 reads
+// a byte of face bits per voxel (bit f: a pore voxel whose component touches face f; the
+// boundary-connected volumes of getReactiveAreaDensityTensor) and carries the six per-face
+// counts packed in nibbles, so one pass yields all six surfaces.
+//
+// Per cube the arithmetic is the reference's float arithmetic, operation by operation and
+// without contraction (the file is built with -ffp-contract=off): positions (1/N)*i and
+// (1/N)*i + 1/N, vertexInterp / lerp (VolumeSurface.cu:171-178, CudaMath.h:1134-1137),
+// triangleArea = 0.5 * length(cross(v1 - v0, v2 - v0)) (VolumeSurface.cu:187-189), the
+// cube's triangle areas summed in double in table order (VolumeSurface.cu:240-255).  The
+// volume sum is double-double with one final rounding, so it does not depend on the
+// reduction tree.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "mc_table.h"
+
+namespace fta {
+
+constexpr int kTX = 64, kTY = 4, kTZ = 16, kNT = kTX * kTY;
+constexpr int kLX = kTX + 1, kLY = kTY + 1, kNL = kLX * kLY;   // lattice points of a tile
+constexpr int kVX = kTX + 2, kVY = kTY + 2, kNV = kVX * kVY;   // voxels of a tile plane with halo
+
+__constant__ uint64_t c_tri[256] = FAST_MC_TRI_WORDS;
+
+// Double-double sum of doubles (TwoSum, then renormalise).
+struct DdSum {
+    double hi, lo;
+    __device__ __forceinline__ void init() { hi = 0.0; lo = 0.0; }
+    __device__ __forceinline__ void add2(double p, double e) {
+        const double s = hi + p;
+        const double v = s - hi;
+        double t = (hi - (s - v)) + (p - v);
+        t += lo + e;
+        hi = s + t;
+        lo = t - (hi - s);
+    }
+};
+
+__device__ __forceinline__ void wave_combine(DdSum& a) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double h = __shfl_xor(a.hi, off, 64);
+        const double l = __shfl_xor(a.lo, off, 64);
+        a.add2(h, l);
+    }
+}
+
+// Face bits 0..5 of a code byte, one per nibble.
+__device__ __forceinline__ uint32_t spread6(uint32_t b) {
+    return (b & 1u) | ((b & 2u) << 3) | ((b & 4u) << 6) | ((b & 8u) << 9) | ((b & 16u) << 12) | ((b & 32u) << 15);
+}
+
+// Corner coordinates of one cube: X0 = (1/nx) i, X1 = X0 + 1/nx, likewise y and z.
+struct CubeGeo {
+    float X0, X1, Y0, Y1, Z0, Z1;
+};
+
+// vertexInterp along one axis: the other two coordinates of the edge's end points are equal,
+// so p0 + t (p1 - p0) leaves them unchanged.
+__device__ __forceinline__ float lerp_axis(float iso, float fa, float fb, float A, float B) {
+    if (fb == fa) return A;
+    const float t = (iso - fa) / (fb - fa);
+    return A + t * (B - A);
+}
+
+// Vertex on edge e (corners a -> b as in the table's edge list).
+__device__ __forceinline__ void edge_vertex(unsigned e, const float (&f)[8], const CubeGeo& g, float iso, float& x,
+                                            float& y, float& z) {
+    switch (e) {
+        case 0: x = lerp_axis(iso, f[0], f[1], g.X0, g.X1); y = g.Y0; z = g.Z0; break;
+        case 1: x = g.X1; y = lerp_axis(iso, f[1], f[2], g.Y0, g.Y1); z = g.Z0; break;
+        case 2: x = lerp_axis(iso, f[2], f[3], g.X1, g.X0); y = g.Y1; z = g.Z0; break;
+        case 3: x = g.X0; y = lerp_axis(iso, f[3], f[0], g.Y1, g.Y0); z = g.Z0; break;
+        case 4: x = lerp_axis(iso, f[4], f[5], g.X0, g.X1); y = g.Y0; z = g.Z1; break;
+        case 5: x = g.X1; y = lerp_axis(iso, f[5], f[6], g.Y0, g.Y1); z = g.Z1; break;
+        case 6: x = lerp_axis(iso, f[6], f[7], g.X1, g.X0); y = g.Y1; z = g.Z1; break;
+        case 7: x = g.X0; y = lerp_axis(iso, f[7], f[4], g.Y1, g.Y0); z = g.Z1; break;
+        case 8: x = g.X0; y = g.Y0; z = lerp_axis(iso, f[0], f[4], g.Z0, g.Z1); break;
+        case 9: x = g.X1; y = g.Y0; z = lerp_axis(iso, f[1], f[5], g.Z0, g.Z1); break;
+        case 10: x = g.X1; y = g.Y1; z = lerp_axis(iso, f[2], f[6], g.Z0, g.Z1); break;
+        default: x = g.X0; y = g.Y1; z = lerp_axis(iso, f[3], f[7], g.Z0, g.Z1); break;
+    }
+}
+
+// Marching-cubes area of one cube with corner values f (table order of the triangles).
+__device__ __forceinline__ double cube_area(const float (&f)[8], const CubeGeo& g, float iso) {
+    unsigned ci = 0;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) ci |= (f[m] < iso ? 1u : 0u) << m;
+    const uint64_t w = c_tri[ci];
+    const int ntri = (int)(w >> 60);
+    double area = 0.0;
+    for (int j = 0; j < ntri; ++j) {
+        float ax, ay, az, bx, by, bz, cx, cy, cz;
+        edge_vertex((unsigned)(w >> (12 * j)) & 15u, f, g, iso, ax, ay, az);
+        edge_vertex((unsigned)(w >> (12 * j + 4)) & 15u, f, g, iso, bx, by, bz);
+        edge_vertex((unsigned)(w >> (12 * j + 8)) & 15u, f, g, iso, cx, cy, cz);
+        const float ux = bx - ax, uy = by - ay, uz = bz - az;
+        const float vx = cx - ax, vy = cy - ay, vz = cz - az;
+        const float nx = uy * vz - uz * vy;
+        const float ny = uz * vx - ux * vz;
+        const float nz = ux * vy - uy * vx;
+        area += (double)(sqrtf(nx * nx + ny * ny + nz * nz) * 0.5f);
+    }
+    return area;
+}
+
+// One block: cubes [x0, x0+64) x [y0, y0+4) x [k0, k0+16).  Writes NS double-double
+// partials (hi, lo) per block to part[(block * NS + s) * 2].
+template <int MODE>
+__global__ __launch_bounds__(kNT) void k_mc_area(const uint8_t* __restrict__ src, int nx, int ny, int nz, float iso,
+                                                 double* __restrict__ part) {
+    constexpr int NS = MODE ? 6 : 1;
+    __shared__ uint32_t V[kNV];
+    __shared__ uint32_t S[kNL];
+    __shared__ uint32_t CP[2][kNL];
+    __shared__ double red[kNT / 64][NS][2];
+
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * kTX, y0 = blockIdx.y * kTY, k0 = blockIdx.z * kTZ;
+    const int k1 = min(k0 + kTZ, nz);
+    const size_t sxy = (size_t)nx * (size_t)ny;
+    const float sx = 1.0f / (float)nx, sy = 1.0f / (float)ny, sz = 1.0f / (float)nz;
+
+    auto load_plane = [&](int z) {
+        const uint8_t* plane = src + sxy * (size_t)z;
+        for (int v = tid; v < kNV; v += kNT) {
+            const int a = min(max(x0 - 1 + v % kVX, 0), nx - 1);
+            const int b = min(max(y0 - 1 + v / kVX, 0), ny - 1);
+            const uint32_t byte = plane[(size_t)a + (size_t)nx * (size_t)b];
+            V[v] = MODE ? spread6(byte) : byte;
+        }
+    };
+    auto lattice_sum = [&](int l) {
+        const int lx = l % kLX, ly = l / kLX;
+        return V[ly * kVX + lx] + V[ly * kVX + lx + 1] + V[(ly + 1) * kVX + lx] + V[(ly + 1) * kVX + lx + 1];
+    };
+
+    // Corner counts of lattice plane k0: voxel planes k0 - 1 and k0 (clamped).
+    load_plane(max(k0 - 1, 0));
+    __syncthreads();
+    for (int l = tid; l < kNL; l += kNT) S[l] = lattice_sum(l);
+    __syncthreads();
+    load_plane(k0);
+    __syncthreads();
+    int cur = 0;
+    for (int l = tid; l < kNL; l += kNT) {
+        const uint32_t s = lattice_sum(l);
+        CP[0][l] = S[l] + s;
+        S[l] = s;
+    }
+
+    const int tx = tid % kTX, ty = tid / kTX;
+    const int i = x0 + tx, j = y0 + ty;
+    const bool active = i < nx && j < ny;
+    CubeGeo g;
+    g.X0 = sx * (float)i;
+    g.X1 = g.X0 + sx;
+    g.Y0 = sy * (float)j;
+    g.Y1 = g.Y0 + sy;
+    DdSum acc[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) acc[s].init();
+
+    for (int k = k0; k < k1; ++k) {
+        __syncthreads();   // CP[cur] complete, V free
+        load_plane(min(k + 1, nz - 1));
+        __syncthreads();
+        for (int l = tid; l < kNL; l += kNT) {
+            const uint32_t s = lattice_sum(l);
+            CP[cur ^ 1][l] = S[l] + s;
+            S[l] = s;
+        }
+        __syncthreads();
+        if (active) {
+            const uint32_t* lo = CP[cur];
+            const uint32_t* hi = CP[cur ^ 1];
+            const int b = ty * kLX + tx;
+            const uint32_t c[8] = {lo[b], lo[b + 1], lo[b + kLX + 1], lo[b + kLX],
+                                   hi[b], hi[b + 1], hi[b + kLX + 1], hi[b + kLX]};
+            g.Z0 = sz * (float)k;
+            g.Z1 = g.Z0 + sz;
+            if (MODE == 0) {
+                float f[8];
+#pragma unroll
+                for (int m = 0; m < 8; ++m) f[m] = (float)c[m] / 2040.0f;
+                acc[0].add2(cube_area(f, g, iso), 0.0);
+            } else {
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    float f[8];
+#pragma unroll
+                    for (int m = 0; m < 8; ++m) f[m] = (float)((c[m] >> (4 * s)) & 15u) / 8.0f;
+                    acc[s].add2(cube_area(f, g, iso), 0.0);
+                }
+            }
+        }
+        cur ^= 1;
+    }
+
+    // Fixed-order block reduction.
+    const int lane = tid & 63, w = tid >> 6;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        DdSum a = acc[s];
+        wave_combine(a);
+        if (lane == 0) {
+            red[w][s][0] = a.hi;
+            red[w][s][1] = a.lo;
+        }
+    }
+    __syncthreads();
+    if (tid < NS) {
+        DdSum a;
+        a.init();
+        for (int q = 0; q < kNT / 64; ++q) a.add2(red[q][tid][0], red[q][tid][1]);
+        const size_t blk = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        part[(blk * NS + tid) * 2] = a.hi;
+        part[(blk * NS + tid) * 2 + 1] = a.lo;
+    }
+}
+
+// One block per surface: sum that surface's nblk block partials in a fixed order.
+__global__ __launch_bounds__(256) void k_mc_final(const double* __restrict__ part, size_t nblk, int ns,
+                                                  double* __restrict__ out) {
+    __shared__ double red[4][2];
+    const int s = blockIdx.x, tid = threadIdx.x;
+    DdSum a;
+    a.init();
+    for (size_t b = tid; b < nblk; b += 256) a.add2(part[(b * ns + s) * 2], part[(b * ns + s) * 2 + 1]);
+    wave_combine(a);
+    if ((tid & 63) == 0) {
+        red[tid >> 6][0] = a.hi;
+        red[tid >> 6][1] = a.lo;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        DdSum t;
+        t.init();
+        for (int q = 0; q < 4; ++q) t.add2(red[q][0], red[q][1]);
+        out[s] = t.hi + t.lo;
+    }
+}
+
+// Grid of the area kernel for an nx x ny x nz volume.
+dim3 mc_grid(int nx, int ny, int nz) {
+    return dim3((unsigned)((nx + kTX - 1) / kTX), (unsigned)((ny + kTY - 1) / kTY), (unsigned)((nz + kTZ - 1) / kTZ));
+}
+
+size_t mc_blocks(int nx, int ny, int nz) {
+    const dim3 g = mc_grid(nx, ny, nz);
+    return (size_t)g.x * g.y * g.z;
+}
+
+// Areas of `src` (an nx x ny x nz volume): MODE 0 one surface of the bytes, MODE 1 six
+// surfaces of face bits.  part: mc_blocks() * NS * 2 doubles; out: NS doubles (device).
+hipError_t mc_area(const uint8_t* src, int nx, int ny, int nz, float iso, int faces, double* part, double* out,
+                   hipStream_t s) {
+    const dim3 grid = mc_grid(nx, ny, nz);
+    const size_t nblk = mc_blocks(nx, ny, nz);
+    if (faces) {
+        hipLaunchKernelGGL(k_mc_area<1>, grid, dim3(kNT), 0, s, src, nx, ny, nz, iso, part);
+        hipLaunchKernelGGL(k_mc_final, dim3(6), dim3(256), 0, s, part, nblk, 6, out);
+    } else {
+        hipLaunchKernelGGL(k_mc_area<0>, grid, dim3(kNT), 0, s, src, nx, ny, nz, iso, part);
+        hipLaunchKernelGGL(k_mc_final, dim3(1), dim3(256), 0, s, part, nblk, 1, out);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace fta

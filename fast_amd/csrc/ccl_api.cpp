@@ -16,9 +16,7 @@ hipError_t ccl_reachable(const uint32_t* L, long long n, const uint8_t* table, u
                          hipStream_t s);
 }  // namespace ftc
 
-using namespace fth;
-
-namespace {
+namespace fth {
 
 // Label the volume (cached on the volume per background value).  Returns numLabels = K + 1.
 int ensure_labels(fast_volume* v, uint8_t background, uint32_t* num_labels) {
@@ -64,7 +62,9 @@ int face_table(fast_volume* v, uint32_t numLabels, uint8_t** out) {
     return FAST_OK;
 }
 
-}  // namespace
+}  // namespace fth
+
+using namespace fth;
 
 extern "C" {
 

@@ -230,6 +230,20 @@ __global__ __launch_bounds__(256) void k_reachable(const uint32_t* L, long long 
     }
 }
 
+// Face bits of every voxel: bit f set when its label touches face f (rows 0..5 of the face
+// table; label 0, the background, is on no row).  One byte per voxel for the six
+// boundary-connected volumes of generateBoundaryConnectedVolume (VolumeSegmentation.cpp:66-90).
+__global__ __launch_bounds__(256) void k_face_codes(const uint32_t* L, long long n, const uint8_t* table,
+                                                    uint32_t numLabels, uint8_t* out) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const uint32_t l = L[i];
+        uint32_t bits = 0;
+#pragma unroll
+        for (int f = 0; f < 6; ++f) bits |= (uint32_t)(table[(size_t)f * numLabels + l] != 0) << f;
+        out[i] = (uint8_t)bits;
+    }
+}
+
 inline unsigned grid_for(long long n) {
     long long b = (n + 255) / 256;
     return (unsigned)(b > 65536 ? 65536 : (b < 1 ? 1 : b));
@@ -267,6 +281,12 @@ hipError_t ccl_reachable(const uint32_t* L, long long n, const uint8_t* table, u
                          hipStream_t s) {
     const int k = dir / 2;
     hipLaunchKernelGGL(k_reachable, dim3(grid_for(n)), dim3(256), 0, s, L, n, table, numLabels, 2 * k, 2 * k + 1, out);
+    return hipGetLastError();
+}
+
+hipError_t ccl_face_codes(const uint32_t* L, long long n, const uint8_t* table, uint32_t numLabels, uint8_t* out,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_face_codes, dim3(grid_for(n)), dim3(256), 0, s, L, n, table, numLabels, out);
     return hipGetLastError();
 }
 

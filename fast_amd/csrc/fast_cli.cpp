@@ -1,6 +1,7 @@
-// `fast tau` command line on top of libfasttau's C ABI.
+// `fast tau` and `fast alpha` command line on top of libfasttau's C ABI.
 //
 // Mirrors the reference CLI (fast/src/main.cpp:20-134, fast/src/TauModule.h:12-164,
+// fast/src/AlphaModule.h:8-219,
 // fast/src/VolumeIOModule.h, fast/src/CSVOutputModule.h): the same flags, the same
 // direction strings (fast/src/utility.h:84-113), one CSV row per direction with the same
 // key-sorted columns and default 6-significant-digit formatting, header only when the
@@ -10,6 +11,10 @@
 // (VolumeIOModule.h:153-245).  Additions: `-p mixed` (FP32 storage / FP64 arithmetic),
 // `--reach FILE` (reachable-pore mask of the first direction's axis as .bin) and `--gpu N`.
 // `--cpu` is not offered: this library has no CPU solver; `.pos` inputs are not read.
+// `fast alpha` writes the reference's rows (AlphaModule::getAlpha, AlphaModule.h:28-174):
+// per direction the boundary-connected area of that face, then a DIR_NONE row with the
+// whole volume's area; with --basic, the whole volume's area on every direction's row.
+// Its --mesh (OBJ export) and --segmentinfo (segment CSV) outputs are not offered.
 #include "fast_tau.h"
 
 #include <algorithm>
@@ -126,7 +131,7 @@ bool crop(std::vector<uint8_t>& m, int (&dim)[3], const int (&o)[3], const int (
 }
 
 void usage() {
-    std::cout << "Usage: fast tau <input> [options]   (input: .bin file, folder of .tif/.tiff slices, .sph)\n"
+    std::cout << "Usage: fast tau|alpha <input> [options]   (input: .bin file, folder of .tif/.tiff slices, .sph)\n"
                  "  -d, --dir STR        x|y|z (combinable), all, pos, neg, xneg, yneg, zneg (default x-)\n"
                  "  --solver CG|BICGSTAB  (default CG)\n"
                  "  --tol N              tolerance 1e-N (default 6)\n"
@@ -140,7 +145,64 @@ void usage() {
                  "  --sub X|X,Y,Z        subvolume size (and the .sph resolution)\n"
                  "  --origin X|X,Y,Z     subvolume origin\n"
                  "  --randsub X|X,Y,Z    subvolume of this size at a random origin\n"
-                 "  --aspect X|X,Y,Z     aspect-ratio crop; one component must be 1.0\n";
+                 "  --aspect X|X,Y,Z     aspect-ratio crop; one component must be 1.0\n"
+                 "alpha:\n"
+                 "  --basic              whole-volume area only (no connectivity)\n";
+}
+
+// AlphaModule::getAlpha (AlphaModule.h:28-174): porosity as getPorosity<float>, the
+// per-face tensor (CCL included in its time, a sixth per row), then the DIR_NONE row.
+int run_alpha(fast_ctx* ctx, fast_volume* vol, const std::string& input, const int (&dim)[3], uint64_t zeros,
+              bool basic, const std::vector<int>& dirs, const std::string& output) {
+    CsvOut out(output);
+    Row o;
+    o["ID"] = input;
+    o["dimx"] = fmt(dim[0]);
+    o["dimy"] = fmt(dim[1]);
+    o["dimz"] = fmt(dim[2]);
+    const size_t n = (size_t)dim[0] * dim[1] * dim[2];
+    o["porosity"] = fmt((float)zeros / (float)n);
+    auto timed = [](auto&& f) {
+        const auto t0 = std::chrono::system_clock::now();
+        const int rc = f();
+        return std::make_pair(rc, std::chrono::duration<double>(std::chrono::system_clock::now() - t0).count());
+    };
+    double whole = 0.0;
+    if (!basic) {
+        double tensor[6];
+        auto [rc, dt] = timed([&] { return fast_alpha_area_tensor(vol, 0.5f, tensor); });
+        if (rc != FAST_OK) {
+            std::cerr << "alpha failed: " << fast_ctx_last_error(ctx) << std::endl;
+            return 1;
+        }
+        for (int d : dirs) {
+            o["rad"] = fmt(tensor[d]);
+            o["radTime"] = fmt(dt / 6.0f);
+            o["dir"] = kDirNames[d];
+            out.add(o);
+        }
+        auto [rc2, dt2] = timed([&] { return fast_alpha_area(vol, 0.5f, &whole); });
+        if (rc2 != FAST_OK) {
+            std::cerr << "alpha failed: " << fast_ctx_last_error(ctx) << std::endl;
+            return 1;
+        }
+        o["rad"] = fmt((float)whole);
+        o["radTime"] = fmt(dt2);
+        o["dir"] = "DIR_NONE";
+        out.add(o);
+        return 0;
+    }
+    auto [rc, dt] = timed([&] { return fast_alpha_area(vol, 0.5f, &whole); });
+    if (rc != FAST_OK) {
+        std::cerr << "alpha failed: " << fast_ctx_last_error(ctx) << std::endl;
+        return 1;
+    }
+    for (size_t k = 0; k < dirs.size(); ++k) {
+        o["radBasic"] = fmt((float)whole);
+        o["radBasicTime"] = fmt(dt);
+        out.add(o);
+    }
+    return 0;
 }
 
 }  // namespace
@@ -155,7 +217,8 @@ int main(int argc, char** argv) {
         usage();
         return 0;
     }
-    if (action != "tau") {
+    const bool alpha = (action == "alpha");
+    if (action != "tau" && !alpha) {
         std::cerr << "Unknown action." << std::endl;
         usage();
         return 1;
@@ -166,7 +229,7 @@ int main(int argc, char** argv) {
     bool verbose = false;
     int sub[3] = {0, 0, 0}, origin[3] = {0, 0, 0}, randsub[3] = {0, 0, 0};
     float aspect[3] = {1.0f, 1.0f, 1.0f};
-    bool has_sub = false, has_randsub = false, has_aspect = false;
+    bool has_sub = false, has_randsub = false, has_aspect = false, basic = false;
     for (int i = 2; i < argc; ++i) {
         const std::string a = argv[i];
         auto next = [&](const char* what) -> std::string {
@@ -198,6 +261,11 @@ int main(int argc, char** argv) {
             (void)next("--cputhreads");
         } else if (a == "--cpu") {
             std::cerr << "--cpu: this build has no CPU solver; the tau solve runs on the GPU" << std::endl;
+            return 1;
+        } else if (alpha && a == "--basic") {
+            basic = true;
+        } else if (alpha && (a == "--mesh" || a == "--segmentinfo")) {
+            std::cerr << a << ": not offered by this build" << std::endl;
             return 1;
         } else if (a == "--reach") {
             reach = next("--reach");
@@ -330,6 +398,12 @@ int main(int argc, char** argv) {
     }
     uint64_t zeros = 0;
     fast_volume_count_zero(vol, &zeros);
+    if (alpha) {
+        const int status = run_alpha(ctx, vol, input, dim, zeros, basic, get_dirs(dirs), output);
+        fast_volume_destroy(vol);
+        fast_ctx_destroy(ctx);
+        return status;
+    }
     const float porosity = (float)((double)zeros / (double)mask.size());   // TauModule.h:62
 
     fast_tau_params p;

@@ -79,6 +79,11 @@ struct Job {
     Shape shape{64, 8, 16};
 };
 
+// CCL on the volume's mask (ccl_api.cpp): labels cached on the volume per background value
+// (numLabels = K + 1), and the 7-row per-face label table in device memory (caller frees).
+int ensure_labels(fast_volume* v, uint8_t background, uint32_t* num_labels);
+int face_table(fast_volume* v, uint32_t numLabels, uint8_t** out);
+
 void set_error(fast_ctx* ctx, const char* fmt, ...);
 void* dalloc(fast_ctx* ctx, size_t bytes);
 void dfree(fast_ctx* ctx, void* p);

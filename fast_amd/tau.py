@@ -214,6 +214,21 @@ class Volume:
               self.ctx.handle)
         return out, int(nl.value)
 
+    def area(self, isovalue: float = 0.5) -> float:
+        """Marching-cubes area of the mask's isovalue level set in unit-cube coordinates
+        (getVolumeArea + Volume::sum, VolumeSurface.cpp:9-36, VolumeMeasures.cpp:294-300)."""
+        out = C.c_double()
+        check(lib().fast_alpha_area(self._h, float(isovalue), C.byref(out)), "fast_alpha_area", self.ctx.handle)
+        return float(out.value)
+
+    def area_tensor(self, isovalue: float = 0.5) -> np.ndarray:
+        """Per-face areas (Dir order) of the pore space connected to each face
+        (getReactiveAreaDensityTensor, VolumeMeasures.cpp:303-333)."""
+        out = np.empty(6, dtype=np.float64)
+        check(lib().fast_alpha_area_tensor(self._h, float(isovalue), out.ctypes.data_as(C.POINTER(C.c_double))),
+              "fast_alpha_area_tensor", self.ctx.handle)
+        return out
+
     def apply(self, d: int, x: np.ndarray, precision="double") -> np.ndarray:
         """y = A x with the solver's matrix-free operator (diagnostic)."""
         if _as_precision(precision) == Precision.FP32:
@@ -303,6 +318,30 @@ def getTortuosity(mask, params: TortuosityParams, solverType=DiffusionSolverType
     if iterationsOut is not None:
         iterationsOut.append(int(res.iterations))
     return float(res.tau)
+
+
+def getReactiveAreaDensity(mask, isovalue: float = 0.5, ctx: Optional[Context] = None) -> float:
+    """getReactiveAreaDensity<double>(mask, mask.dim(), isovalue) (VolumeMeasures.cpp:294-300):
+    the marching-cubes area of the mask (a [z, y, x] uint8 array or a resident Volume)."""
+    ctx = ctx or (mask.ctx if isinstance(mask, Volume) else default_context())
+    vol = mask if isinstance(mask, Volume) else Volume(ctx, mask)
+    try:
+        return vol.area(isovalue)
+    finally:
+        if vol is not mask:
+            vol.close()
+
+
+def getReactiveAreaDensityTensor(mask, isovalue: float = 0.5, ctx: Optional[Context] = None) -> np.ndarray:
+    """getReactiveAreaDensityTensor<double>(getVolumeCCL(mask, 255)) (VolumeMeasures.cpp:303-333):
+    six areas in Dir order, each of the pore space connected to that face."""
+    ctx = ctx or (mask.ctx if isinstance(mask, Volume) else default_context())
+    vol = mask if isinstance(mask, Volume) else Volume(ctx, mask)
+    try:
+        return vol.area_tensor(isovalue)
+    finally:
+        if vol is not mask:
+            vol.close()
 
 
 def solve_batch(ctx: Context, vols: Sequence[Volume], params: Sequence[TauParams]) -> list:

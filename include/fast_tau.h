@@ -168,6 +168,23 @@ int fast_reachable_mask(fast_volume* vol, int dir, uint8_t background, uint8_t* 
  * num_labels bytes (rows in Dir order, row 6 = present on all faces).  table: 7*num_labels. */
 int fast_ccl_boundary_table(fast_volume* vol, uint8_t background, uint8_t* table, uint32_t table_labels);
 
+/* ---- reactive area density (`fast alpha`) ---------------------------------- */
+/* Marching-cubes area of the `isovalue` level set (0.5 in the reference callers) of the
+ * volume read as its linearly filtered texture: each voxel corner takes the mean of the 8
+ * voxels around it (indices clamped into the volume), byte/255, and a cube corner below the
+ * isovalue is inside.  Coordinates are normalised per axis, so the volume spans the unit
+ * cube and the area is the area density in units of 1/L.  Triangles are Bourke's table;
+ * per-cube areas in the reference's float arithmetic, summed in double (one rounding).
+ * getReactiveAreaDensity<double>(mask, dim, 0.5f) (VolumeMeasures.cpp:294-300) ->
+ * getVolumeArea (VolumeSurface.cpp:9-36) -> VolumeSurface_MarchingCubesArea
+ * (VolumeSurface.cu:203-258, 507-535) and Volume::sum. */
+int fast_alpha_area(fast_volume* vol, float isovalue, double* area);
+/* The six per-face areas of getReactiveAreaDensityTensor<double>(getVolumeCCL(vol, 255))
+ * (VolumeMeasures.cpp:303-333): for face f (Dir order x+, x-, y+, y-, z+, z-), the area
+ * above on the volume that is 255 on the pore voxels whose component touches face f and 0
+ * elsewhere (generateBoundaryConnectedVolume, VolumeSegmentation.cpp:66-90).  area: 6. */
+int fast_alpha_area_tensor(fast_volume* vol, float isovalue, double area[6]);
+
 /* ---- serving: continuous batching ------------------------------------------ */
 /* A request loop that admits tau solves of mixed sizes by HBM budget, keeps masks and
  * Krylov state resident, lets requests join the running batch at iteration boundaries

@@ -65,6 +65,14 @@ def lib():
         h.oracle_boundary_table.argtypes = [u32, C.c_int, C.c_int, C.c_int, C.c_uint32, u8]
         h.oracle_reachable.restype = C.c_uint32
         h.oracle_reachable.argtypes = [u8, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint8, u8, u32]
+        h.oracle_mc_table.restype = None
+        h.oracle_mc_table.argtypes = [C.POINTER(C.c_uint64)]
+        h.oracle_mc_cube_area.restype = C.c_double
+        h.oracle_mc_cube_area.argtypes = [u8, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float]
+        h.oracle_mc_area.restype = C.c_double
+        h.oracle_mc_area.argtypes = [u8, C.c_int, C.c_int, C.c_int, C.c_float]
+        h.oracle_mc_area_tensor.restype = None
+        h.oracle_mc_area_tensor.argtypes = [u8, C.c_int, C.c_int, C.c_int, C.c_float, dp]
         _lib = h
     return _lib
 
@@ -161,3 +169,34 @@ def reachable(mask, d, background=255):
     nl = lib().oracle_reachable(p, nx, ny, nz, int(d), background, out.ctypes.data_as(C.POINTER(C.c_uint8)),
                                 lab.ctypes.data_as(C.POINTER(C.c_uint32)))
     return out, lab, int(nl)
+
+
+def mc_table():
+    """The triangle table as lists of edge ids per cube configuration (256 lists)."""
+    w = (C.c_uint64 * 256)()
+    lib().oracle_mc_table(w)
+    out = []
+    for c in range(256):
+        v = int(w[c])
+        n = v >> 60
+        out.append([(v >> (4 * j)) & 15 for j in range(3 * n)])
+    return out
+
+
+def mc_cube_area(mask, x, y, z, isovalue=0.5):
+    m, p, nx, ny, nz = _m(mask)
+    return lib().oracle_mc_cube_area(p, nx, ny, nz, int(x), int(y), int(z), float(isovalue))
+
+
+def mc_area(mask, isovalue=0.5):
+    """getReactiveAreaDensity: marching-cubes area of the mask (alpha_oracle.cpp)."""
+    m, p, nx, ny, nz = _m(mask)
+    return lib().oracle_mc_area(p, nx, ny, nz, float(isovalue))
+
+
+def mc_area_tensor(mask, isovalue=0.5):
+    """getReactiveAreaDensityTensor over getVolumeCCL(mask, 255): six per-face areas."""
+    m, p, nx, ny, nz = _m(mask)
+    out = np.empty(6, dtype=np.float64)
+    lib().oracle_mc_area_tensor(p, nx, ny, nz, float(isovalue), _dp(out))
+    return out

@@ -95,3 +95,37 @@ def test_cli_sph_and_crops(tmp_path):
 
     r = _run([str(tmp_path / "v.bin"), "--randsub", "20"], tmp_path, "randsub")
     assert (r["dimx"], r["dimy"], r["dimz"]) == ("20", "20", "20") and float(r["tau"]) > 0
+
+
+def _fmt6(v):
+    """A value as the CLI's default ostream prints it (6 significant digits)."""
+    return float(f"{v:.6g}")
+
+
+def test_cli_alpha_rows(tmp_path):
+    """`fast alpha -d all`: one row per face with that face's connected area, then a DIR_NONE
+    row with the whole volume's area (printed as float); `--basic`: the whole area on every
+    row.  Porosity is getPorosity<float>."""
+    m = F.generate_spheres((40, 36, 32), 4.0, 0.5, 3)
+    vin = tmp_path / "v.bin"
+    F.save_bin(str(vin), m)
+    out = tmp_path / "alpha.csv"
+    r = subprocess.run([str(EXE), "alpha", str(vin), "-d", "all", "-o", str(out)], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    rows = list(csv.DictReader(open(out)))
+    assert [row["dir"] for row in rows] == ["X_POS", "X_NEG", "Y_POS", "Y_NEG", "Z_POS", "Z_NEG", "DIR_NONE"]
+    tensor = O.mc_area_tensor(m)
+    for f in range(6):
+        assert float(rows[f]["rad"]) == _fmt6(tensor[f])
+    assert float(rows[6]["rad"]) == _fmt6(float(np.float32(O.mc_area(m))))
+    por = np.float32(np.count_nonzero(m == 0)) / np.float32(m.size)
+    assert all(float(row["porosity"]) == _fmt6(float(por)) for row in rows)
+    out_b = tmp_path / "basic.csv"
+    r = subprocess.run([str(EXE), "alpha", str(vin), "--basic", "-d", "xy", "-o", str(out_b)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rows = list(csv.DictReader(open(out_b)))
+    assert len(rows) == 2 and "dir" not in rows[0]
+    assert all(float(row["radBasic"]) == _fmt6(float(np.float32(O.mc_area(m)))) for row in rows)
