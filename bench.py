@@ -79,8 +79,12 @@ def cpu_baseline(mask: np.ndarray, gpu_iters: float, threads: int, sample_iters:
                 break
     except OSError:
         pass
+    t = time.perf_counter()
+    O.mc_area(mask)
+    t_area = time.perf_counter() - t
     return {
         "value": 1.0 / solve_s, "unit": "solves/s", "cores": threads, "kind": "port",
+        "alpha_area_s": t_area,
         "cpu_model": model, "nproc": os.cpu_count(),
         "sample": (f"256^3 volume, X_POS, CG FP64 (oracle restating the reference --cpu path): assembly "
                    f"{t_setup:.2f} s + {sample_iters} iterations at {per_iter * 1e3:.0f} ms each, extrapolated to "
@@ -171,6 +175,9 @@ def profile_program(kind: str, n: int) -> int:
         rb = (C.c_double * 5)()
         lib().fast_debug_time_rb_passes(vol.handle, 0, 20, rb)
         print(f"mixed cg red-black: P, Y, Q, U ms {list(rb)[:4]} iteration ms {rb[4]}", flush=True)
+        al = (C.c_double * 3)()
+        lib().fast_debug_time_alpha(vol.handle, 5, al)
+        print(f"alpha: area ms {al[0]} tensor (cached labels) ms {al[1]} tensor with CCL ms {al[2]}", flush=True)
         import torch   # calibration kernel of known traffic: read 256 MiB, write 256 MiB
         a = torch.ones(64 * 2 ** 20, dtype=torch.float32, device="cuda")
         b = torch.empty_like(a)
@@ -268,6 +275,14 @@ def main() -> int:
     rb = (C.c_double * 5)()
     lib().fast_debug_time_rb_passes(vol.handle, 0, 20, rb)
     rb_pass = list(rb)
+    # `fast alpha` on the same volume: whole-volume area, six per-face areas from cached
+    # labels, and the six from scratch (CCL included), HIP events on the same stream
+    al = (C.c_double * 3)()
+    lib().fast_debug_time_alpha(vol.handle, 10, al)
+    alpha = {"area_ms": al[0], "tensor_cached_labels_ms": al[1], "tensor_with_ccl_ms": al[2],
+             "area_voxels_per_s": nvox / (al[0] * 1e-3),
+             "area_mask_read_GBs": nvox / (al[0] * 1e-3) / 1e9,
+             "timing": "HIP events around 10 back-to-back runs of each"}
     # algorithmic bytes per launch (each vector element and code byte once): FP32 r, p, x, q
     # and FP64 Y, 1-byte codes, per red (n_r) and black (n_b) voxel
     n_r = (nvox + 1) // 2 if all(e % 2 for e in mask.shape) else nvox // 2
@@ -366,6 +381,7 @@ def main() -> int:
         "mixed_cg_passes_512_dram": dram,
         "pass_ms": {"mixed_cg_red_black": rb_pass, "mixed_cg_full_grid": mixed_full_pass, "fp64_cg": fp64_cg_pass,
                     "fp64_bicgstab": fp64_bicg_pass},
+        "alpha_reactive_area_density": alpha,
         "cpu_baseline": cpu,
         "serving": serve,
     }

@@ -107,6 +107,7 @@ PROTOTYPES = [
     ("fast_ccl_boundary_table", C.c_int, [_vp, C.c_uint8, _u8p, C.c_uint32]),
     ("fast_alpha_area", C.c_int, [_vp, C.c_float, C.POINTER(C.c_double)]),
     ("fast_alpha_area_tensor", C.c_int, [_vp, C.c_float, C.POINTER(C.c_double)]),
+    ("fast_debug_time_alpha", C.c_int, [_vp, C.c_int, C.POINTER(C.c_double)]),
     ("fast_server_create", C.c_int, [_vp, C.POINTER(ServerConfig), C.POINTER(_vp)]),
     ("fast_server_submit", C.c_int, [_vp, _u8p, C.c_int, C.c_int, C.c_int, C.POINTER(TauParams), C.POINTER(C.c_uint64)]),
     ("fast_server_wait", C.c_int, [_vp, C.c_int, C.POINTER(C.c_uint64), C.POINTER(TauResult), _dp]),

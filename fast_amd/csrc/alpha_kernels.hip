@@ -82,38 +82,52 @@ __device__ __forceinline__ float lerp_axis(float iso, float fa, float fb, float 
     return A + t * (B - A);
 }
 
-// Vertex on edge e (corners a -> b as in the table's edge list).
-__device__ __forceinline__ void edge_vertex(unsigned e, const float (&f)[8], const CubeGeo& g, float iso, float& x,
-                                            float& y, float& z) {
-    switch (e) {
-        case 0: x = lerp_axis(iso, f[0], f[1], g.X0, g.X1); y = g.Y0; z = g.Z0; break;
-        case 1: x = g.X1; y = lerp_axis(iso, f[1], f[2], g.Y0, g.Y1); z = g.Z0; break;
-        case 2: x = lerp_axis(iso, f[2], f[3], g.X1, g.X0); y = g.Y1; z = g.Z0; break;
-        case 3: x = g.X0; y = lerp_axis(iso, f[3], f[0], g.Y1, g.Y0); z = g.Z0; break;
-        case 4: x = lerp_axis(iso, f[4], f[5], g.X0, g.X1); y = g.Y0; z = g.Z1; break;
-        case 5: x = g.X1; y = lerp_axis(iso, f[5], f[6], g.Y0, g.Y1); z = g.Z1; break;
-        case 6: x = lerp_axis(iso, f[6], f[7], g.X1, g.X0); y = g.Y1; z = g.Z1; break;
-        case 7: x = g.X0; y = lerp_axis(iso, f[7], f[4], g.Y1, g.Y0); z = g.Z1; break;
-        case 8: x = g.X0; y = g.Y0; z = lerp_axis(iso, f[0], f[4], g.Z0, g.Z1); break;
-        case 9: x = g.X1; y = g.Y0; z = lerp_axis(iso, f[1], f[5], g.Z0, g.Z1); break;
-        case 10: x = g.X1; y = g.Y1; z = lerp_axis(iso, f[2], f[6], g.Z0, g.Z1); break;
-        default: x = g.X0; y = g.Y1; z = lerp_axis(iso, f[3], f[7], g.Z0, g.Z1); break;
-    }
+// Edge e (corners a -> b of the table's edge list) as 5 bits at 5e: bits 0-1 the axis along
+// which it runs, bits 2-4 whether its fixed x, y, z coordinates are the cube's upper ones.
+constexpr uint64_t edge_desc(int axis, int x1, int y1, int z1) {
+    return (uint64_t)(axis | (x1 << 2) | (y1 << 3) | (z1 << 4));
 }
+constexpr uint64_t kEdgeDesc =
+    edge_desc(0, 0, 0, 0) << 0 | edge_desc(1, 1, 0, 0) << 5 | edge_desc(0, 0, 1, 0) << 10 | edge_desc(1, 0, 0, 0) << 15 |
+    edge_desc(0, 0, 0, 1) << 20 | edge_desc(1, 1, 0, 1) << 25 | edge_desc(0, 0, 1, 1) << 30 | edge_desc(1, 0, 0, 1) << 35 |
+    edge_desc(2, 0, 0, 0) << 40 | edge_desc(2, 1, 0, 0) << 45 | edge_desc(2, 1, 1, 0) << 50 | edge_desc(2, 0, 1, 0) << 55;
 
-// Marching-cubes area of one cube with corner values f (table order of the triangles).
-__device__ __forceinline__ double cube_area(const float (&f)[8], const CubeGeo& g, float iso) {
+// Marching-cubes area of one cube with corner values f (table order of the triangles).  The
+// interpolated coordinate of every edge is computed once, without branching on the edge, and
+// kept in this thread's LDS slots ev[12 * kNT] (stride kNT); a triangle vertex is then its
+// edge's slot plus the two fixed coordinates, so lanes of a wave never diverge on edge ids.
+__device__ __forceinline__ double cube_area(const float (&f)[8], const CubeGeo& g, float iso, float* ev) {
     unsigned ci = 0;
 #pragma unroll
     for (int m = 0; m < 8; ++m) ci |= (f[m] < iso ? 1u : 0u) << m;
+    ev[0 * kNT] = lerp_axis(iso, f[0], f[1], g.X0, g.X1);
+    ev[1 * kNT] = lerp_axis(iso, f[1], f[2], g.Y0, g.Y1);
+    ev[2 * kNT] = lerp_axis(iso, f[2], f[3], g.X1, g.X0);
+    ev[3 * kNT] = lerp_axis(iso, f[3], f[0], g.Y1, g.Y0);
+    ev[4 * kNT] = lerp_axis(iso, f[4], f[5], g.X0, g.X1);
+    ev[5 * kNT] = lerp_axis(iso, f[5], f[6], g.Y0, g.Y1);
+    ev[6 * kNT] = lerp_axis(iso, f[6], f[7], g.X1, g.X0);
+    ev[7 * kNT] = lerp_axis(iso, f[7], f[4], g.Y1, g.Y0);
+    ev[8 * kNT] = lerp_axis(iso, f[0], f[4], g.Z0, g.Z1);
+    ev[9 * kNT] = lerp_axis(iso, f[1], f[5], g.Z0, g.Z1);
+    ev[10 * kNT] = lerp_axis(iso, f[2], f[6], g.Z0, g.Z1);
+    ev[11 * kNT] = lerp_axis(iso, f[3], f[7], g.Z0, g.Z1);
     const uint64_t w = c_tri[ci];
     const int ntri = (int)(w >> 60);
+    auto vertex = [&](unsigned e, float& x, float& y, float& z) {
+        const unsigned d = (unsigned)(kEdgeDesc >> (5 * e)) & 31u;
+        const float v = ev[e * kNT];
+        const unsigned axis = d & 3u;
+        x = axis == 0u ? v : ((d & 4u) ? g.X1 : g.X0);
+        y = axis == 1u ? v : ((d & 8u) ? g.Y1 : g.Y0);
+        z = axis == 2u ? v : ((d & 16u) ? g.Z1 : g.Z0);
+    };
     double area = 0.0;
     for (int j = 0; j < ntri; ++j) {
         float ax, ay, az, bx, by, bz, cx, cy, cz;
-        edge_vertex((unsigned)(w >> (12 * j)) & 15u, f, g, iso, ax, ay, az);
-        edge_vertex((unsigned)(w >> (12 * j + 4)) & 15u, f, g, iso, bx, by, bz);
-        edge_vertex((unsigned)(w >> (12 * j + 8)) & 15u, f, g, iso, cx, cy, cz);
+        vertex((unsigned)(w >> (12 * j)) & 15u, ax, ay, az);
+        vertex((unsigned)(w >> (12 * j + 4)) & 15u, bx, by, bz);
+        vertex((unsigned)(w >> (12 * j + 8)) & 15u, cx, cy, cz);
         const float ux = bx - ax, uy = by - ay, uz = bz - az;
         const float vx = cx - ax, vy = cy - ay, vz = cz - az;
         const float nx = uy * vz - uz * vy;
@@ -126,14 +140,19 @@ __device__ __forceinline__ double cube_area(const float (&f)[8], const CubeGeo& 
 
 // One block: cubes [x0, x0+64) x [y0, y0+4) x [k0, k0+16).  Writes NS double-double
 // partials (hi, lo) per block to part[(block * NS + s) * 2].
+// thr: the smallest corner count whose value (count / 2040, or / 8 for face bits) is not
+// below the isovalue.  The value is monotone in the count, so "corner inside" is count < thr;
+// a cube whose 8 corners are all inside or all outside has no triangles and is skipped
+// without computing its values.
 template <int MODE>
 __global__ __launch_bounds__(kNT) void k_mc_area(const uint8_t* __restrict__ src, int nx, int ny, int nz, float iso,
-                                                 double* __restrict__ part) {
+                                                 uint32_t thr, double* __restrict__ part) {
     constexpr int NS = MODE ? 6 : 1;
     __shared__ uint32_t V[kNV];
     __shared__ uint32_t S[kNL];
     __shared__ uint32_t CP[2][kNL];
     __shared__ double red[kNT / 64][NS][2];
+    __shared__ float EV[12 * kNT];   // per-thread edge coordinates of the cube being triangulated
 
     const int tid = threadIdx.x;
     const int x0 = blockIdx.x * kTX, y0 = blockIdx.y * kTY, k0 = blockIdx.z * kTZ;
@@ -200,17 +219,31 @@ __global__ __launch_bounds__(kNT) void k_mc_area(const uint8_t* __restrict__ src
             g.Z0 = sz * (float)k;
             g.Z1 = g.Z0 + sz;
             if (MODE == 0) {
-                float f[8];
+                unsigned in = 0;
 #pragma unroll
-                for (int m = 0; m < 8; ++m) f[m] = (float)c[m] / 2040.0f;
-                acc[0].add2(cube_area(f, g, iso), 0.0);
-            } else {
-#pragma unroll
-                for (int s = 0; s < NS; ++s) {
+                for (int m = 0; m < 8; ++m) in |= (c[m] < thr ? 1u : 0u) << m;
+                if (in != 0u && in != 255u) {
                     float f[8];
 #pragma unroll
-                    for (int m = 0; m < 8; ++m) f[m] = (float)((c[m] >> (4 * s)) & 15u) / 8.0f;
-                    acc[s].add2(cube_area(f, g, iso), 0.0);
+                    for (int m = 0; m < 8; ++m) f[m] = (float)c[m] / 2040.0f;
+                    acc[0].add2(cube_area(f, g, iso, EV + tid), 0.0);
+                }
+            } else {
+                uint32_t diff = 0;
+#pragma unroll
+                for (int m = 1; m < 8; ++m) diff |= c[m] ^ c[0];
+                if (diff) {   // some face's corners differ
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) {
+                        unsigned in = 0;
+#pragma unroll
+                        for (int m = 0; m < 8; ++m) in |= (((c[m] >> (4 * s)) & 15u) < thr ? 1u : 0u) << m;
+                        if (in == 0u || in == 255u) continue;
+                        float f[8];
+#pragma unroll
+                        for (int m = 0; m < 8; ++m) f[m] = (float)((c[m] >> (4 * s)) & 15u) / 8.0f;
+                        acc[s].add2(cube_area(f, g, iso, EV + tid), 0.0);
+                    }
                 }
             }
         }
@@ -266,6 +299,12 @@ dim3 mc_grid(int nx, int ny, int nz) {
     return dim3((unsigned)((nx + kTX - 1) / kTX), (unsigned)((ny + kTY - 1) / kTY), (unsigned)((nz + kTZ - 1) / kTZ));
 }
 
+// The grid's y and z extents stay within the launch limit (65535).
+bool mc_fits(int nx, int ny, int nz) {
+    const dim3 g = mc_grid(nx, ny, nz);
+    return g.y <= 65535u && g.z <= 65535u;
+}
+
 size_t mc_blocks(int nx, int ny, int nz) {
     const dim3 g = mc_grid(nx, ny, nz);
     return (size_t)g.x * g.y * g.z;
@@ -277,11 +316,16 @@ hipError_t mc_area(const uint8_t* src, int nx, int ny, int nz, float iso, int fa
                    hipStream_t s) {
     const dim3 grid = mc_grid(nx, ny, nz);
     const size_t nblk = mc_blocks(nx, ny, nz);
+    // first count whose value is not below iso (the same float division as the kernels)
+    const uint32_t maxc = faces ? 8u : 2040u;
+    const float div = faces ? 8.0f : 2040.0f;
+    uint32_t thr = 0;
+    while (thr <= maxc && (float)thr / div < iso) ++thr;
     if (faces) {
-        hipLaunchKernelGGL(k_mc_area<1>, grid, dim3(kNT), 0, s, src, nx, ny, nz, iso, part);
+        hipLaunchKernelGGL(k_mc_area<1>, grid, dim3(kNT), 0, s, src, nx, ny, nz, iso, thr, part);
         hipLaunchKernelGGL(k_mc_final, dim3(6), dim3(256), 0, s, part, nblk, 6, out);
     } else {
-        hipLaunchKernelGGL(k_mc_area<0>, grid, dim3(kNT), 0, s, src, nx, ny, nz, iso, part);
+        hipLaunchKernelGGL(k_mc_area<0>, grid, dim3(kNT), 0, s, src, nx, ny, nz, iso, thr, part);
         hipLaunchKernelGGL(k_mc_final, dim3(1), dim3(256), 0, s, part, nblk, 1, out);
     }
     return hipGetLastError();

@@ -1,5 +1,6 @@
 /*
- * fast_tau.h -- C ABI of libfasttau, the MI355X-native tortuosity (`fast tau`) solver.
+ * fast_tau.h -- C ABI of libfasttau, the MI355X-native tortuosity (`fast tau`) solver and
+ * reactive area density (`fast alpha`).
  *
  * The reference exposes this path as a C++ template ABI of libfastlib.so:
  *   template<T> T fast::getTortuosity(const Volume&, const TortuosityParams&,
@@ -8,6 +9,9 @@
  *       fastlib/src/volume/VolumeMeasures.cpp:42-285
  *   template<T> T fast::getPorosity(const Volume&)       VolumeMeasures.h:60-61
  *   fast::getVolumeCCL / generateBoundaryConnectedVolume  VolumeSegmentation.h:33-47
+ *   template<T> T fast::getReactiveAreaDensity(const Volume&, ivec3, float),
+ *   template<T> std::array<T,6> fast::getReactiveAreaDensityTensor(const VolumeCCL&, ivec3, float)
+ *                                                          VolumeMeasures.h:64-77
  *   fast::loadVolumeBinary / saveVolumeBinary              VolumeIO.h:15, VolumeIO.cpp:256-305
  * Every entry point below names the reference interface it replaces.
  *
@@ -276,6 +280,11 @@ int fast_debug_time_passes(fast_volume* vol, int dir, int precision, int solver,
  * update, r.r, r.D^-1 r), each launched `iters` times back to back; ms[4] = mean period of a
  * whole iteration.  ms: 5 doubles. */
 int fast_debug_time_rb_passes(fast_volume* vol, int dir, int iters, double* ms);
+/* `fast alpha` kernels, each run `iters` times back to back on the context's stream (HIP
+ * events): ms[0] = whole-volume area (area pass + final sum), ms[1] = the six per-face areas
+ * from the cached labels (face codes + area pass + final sums), ms[2] = the six per-face
+ * areas from scratch (CCL labels, face table, face codes, area).  ms: 3 doubles. */
+int fast_debug_time_alpha(fast_volume* vol, int iters, double* ms);
 
 #ifdef __cplusplus
 }

@@ -54,12 +54,12 @@ def test_area_and_tensor_equal_the_oracle(ctx, name, make):
 
 
 def test_arbitrary_bytes_and_other_isovalues(ctx):
-    """Corner value = (sum of the 8 bytes) / 2040 for any bytes; isovalues 0.25, 0.5, 0.75."""
+    """Corner value = (sum of the 8 bytes) / 2040 for any bytes; isovalues 0 to 1."""
     rng = np.random.default_rng(11)
     for shape in [(23, 17, 70), (9, 33, 5)]:
         m = rng.integers(0, 256, size=shape, dtype=np.uint8)
         vol = ctx.volume(m)
-        for iso in (0.25, 0.5, 0.75):
+        for iso in (0.0, 0.25, 0.5, 0.75, 1.0):
             assert vol.area(iso) == O.mc_area(m, iso), (shape, iso)
         vol.close()
     m = rand_mask((40, 30, 50), 0.5, 12)
