@@ -141,20 +141,29 @@ __device__ __forceinline__ double cube_area(const float (&f)[8], const CubeGeo& 
 // One block: cubes [x0, x0+64) x [y0, y0+4) x [k0, k0+16).  Writes NS double-double
 // partials (hi, lo) per block to part[(block * NS + s) * 2].
 // thr: the smallest corner count whose value (count / 2040, or / 8 for face bits) is not
-// below the isovalue.  The value is monotone in the count, so "corner inside" is count < thr;
-// a cube whose 8 corners are all inside or all outside has no triangles and is skipped
-// without computing its values.
+// below the isovalue.  The value is monotone in the count, so "corner inside" is count < thr.
+// Per layer every thread classifies its cube with integer work only; cubes with corners on
+// both sides (per face in MODE 1) are queued in LDS (wave ballot + one LDS atomic per wave).
+// Once the queue holds a block's worth of cubes (or at the last layer) the whole block
+// triangulates it densely, so the float work runs on full waves instead of on the few
+// surface lanes of every wave.
 template <int MODE>
 __global__ __launch_bounds__(kNT) void k_mc_area(const uint8_t* __restrict__ src, int nx, int ny, int nz, float iso,
                                                  uint32_t thr, double* __restrict__ part) {
     constexpr int NS = MODE ? 6 : 1;
+    constexpr int QW = MODE ? 1 : 4;          // words of corner counts per queued cube
+    constexpr int KQ = kNT + NS * kNT;        // flush at >= kNT items; a layer adds <= NS per thread
     __shared__ uint32_t V[kNV];
     __shared__ uint32_t S[kNL];
     __shared__ uint32_t CP[2][kNL];
     __shared__ double red[kNT / 64][NS][2];
-    __shared__ float EV[12 * kNT];   // per-thread edge coordinates of the cube being triangulated
+    __shared__ float EV[12 * kNT];            // per-thread edge coordinates of the cube being triangulated
+    __shared__ uint32_t QC[QW][KQ];           // queued corner counts (MODE 0: 2 x 16 bits, MODE 1: 8 nibbles)
+    __shared__ uint32_t QH[KQ];               // queued cube: thread of the tile | surface << 8 | layer << 11
+    __shared__ int qn;
 
     const int tid = threadIdx.x;
+    const int lane = tid & 63;
     const int x0 = blockIdx.x * kTX, y0 = blockIdx.y * kTY, k0 = blockIdx.z * kTZ;
     const int k1 = min(k0 + kTZ, nz);
     const size_t sxy = (size_t)nx * (size_t)ny;
@@ -173,6 +182,20 @@ __global__ __launch_bounds__(kNT) void k_mc_area(const uint8_t* __restrict__ src
         const int lx = l % kLX, ly = l / kLX;
         return V[ly * kVX + lx] + V[ly * kVX + lx + 1] + V[(ly + 1) * kVX + lx] + V[(ly + 1) * kVX + lx + 1];
     };
+    // Append this lane's item when `pred`; called by every lane of the wave.
+    auto append = [&](bool pred, const uint32_t (&words)[QW], uint32_t head) {
+        const unsigned long long mask = __ballot(pred);
+        if (mask == 0ull) return;
+        int base = 0;
+        if (lane == 0) base = atomicAdd(&qn, (int)__popcll(mask));
+        base = __shfl(base, 0, 64);
+        if (pred) {
+            const int slot = base + (int)__popcll(mask & ((1ull << lane) - 1ull));
+#pragma unroll
+            for (int w = 0; w < QW; ++w) QC[w][slot] = words[w];
+            QH[slot] = head;
+        }
+    };
 
     // Corner counts of lattice plane k0: voxel planes k0 - 1 and k0 (clamped).
     load_plane(max(k0 - 1, 0));
@@ -189,19 +212,16 @@ __global__ __launch_bounds__(kNT) void k_mc_area(const uint8_t* __restrict__ src
     }
 
     const int tx = tid % kTX, ty = tid / kTX;
-    const int i = x0 + tx, j = y0 + ty;
-    const bool active = i < nx && j < ny;
-    CubeGeo g;
-    g.X0 = sx * (float)i;
-    g.X1 = g.X0 + sx;
-    g.Y0 = sy * (float)j;
-    g.Y1 = g.Y0 + sy;
+    const bool active = x0 + tx < nx && y0 + ty < ny;
     DdSum acc[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) acc[s].init();
 
+    if (tid == 0) qn = 0;
+    bool flushed = false;
     for (int k = k0; k < k1; ++k) {
-        __syncthreads();   // CP[cur] complete, V free
+        __syncthreads();   // CP[cur] complete; V free; a flushed queue is no longer read
+        if (tid == 0 && flushed) qn = 0;
         load_plane(min(k + 1, nz - 1));
         __syncthreads();
         for (int l = tid; l < kNL; l += kNT) {
@@ -210,48 +230,85 @@ __global__ __launch_bounds__(kNT) void k_mc_area(const uint8_t* __restrict__ src
             S[l] = s;
         }
         __syncthreads();
+        // classify this thread's cube (integer work only) and queue its surfaces
+        uint32_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         if (active) {
             const uint32_t* lo = CP[cur];
             const uint32_t* hi = CP[cur ^ 1];
             const int b = ty * kLX + tx;
-            const uint32_t c[8] = {lo[b], lo[b + 1], lo[b + kLX + 1], lo[b + kLX],
-                                   hi[b], hi[b + 1], hi[b + kLX + 1], hi[b + kLX]};
-            g.Z0 = sz * (float)k;
-            g.Z1 = g.Z0 + sz;
-            if (MODE == 0) {
-                unsigned in = 0;
+            c[0] = lo[b];
+            c[1] = lo[b + 1];
+            c[2] = lo[b + kLX + 1];
+            c[3] = lo[b + kLX];
+            c[4] = hi[b];
+            c[5] = hi[b + 1];
+            c[6] = hi[b + kLX + 1];
+            c[7] = hi[b + kLX];
+        }
+        if constexpr (MODE == 0) {
+            unsigned in = 0;
 #pragma unroll
-                for (int m = 0; m < 8; ++m) in |= (c[m] < thr ? 1u : 0u) << m;
-                if (in != 0u && in != 255u) {
-                    float f[8];
+            for (int m = 0; m < 8; ++m) in |= (c[m] < thr ? 1u : 0u) << m;
+            const uint32_t words[QW] = {c[0] | c[1] << 16, c[2] | c[3] << 16, c[4] | c[5] << 16, c[6] | c[7] << 16};
+            append(active && in != 0u && in != 255u, words, (uint32_t)tid | (uint32_t)(k - k0) << 11);
+        } else {
+            uint32_t diff = 0;
 #pragma unroll
-                    for (int m = 0; m < 8; ++m) f[m] = (float)c[m] / 2040.0f;
-                    acc[0].add2(cube_area(f, g, iso, EV + tid), 0.0);
-                }
-            } else {
-                uint32_t diff = 0;
+            for (int m = 1; m < 8; ++m) diff |= c[m] ^ c[0];
+            if (__ballot(active && diff != 0u) != 0ull) {   // some cube of the wave has differing corners
 #pragma unroll
-                for (int m = 1; m < 8; ++m) diff |= c[m] ^ c[0];
-                if (diff) {   // some face's corners differ
+                for (int s = 0; s < NS; ++s) {
+                    uint32_t nib = 0;
+                    unsigned in = 0;
 #pragma unroll
-                    for (int s = 0; s < NS; ++s) {
-                        unsigned in = 0;
-#pragma unroll
-                        for (int m = 0; m < 8; ++m) in |= (((c[m] >> (4 * s)) & 15u) < thr ? 1u : 0u) << m;
-                        if (in == 0u || in == 255u) continue;
-                        float f[8];
-#pragma unroll
-                        for (int m = 0; m < 8; ++m) f[m] = (float)((c[m] >> (4 * s)) & 15u) / 8.0f;
-                        acc[s].add2(cube_area(f, g, iso, EV + tid), 0.0);
+                    for (int m = 0; m < 8; ++m) {
+                        const uint32_t q = (c[m] >> (4 * s)) & 15u;
+                        nib |= q << (4 * m);
+                        in |= (q < thr ? 1u : 0u) << m;
                     }
+                    const uint32_t words[QW] = {nib};
+                    append(active && in != 0u && in != 255u, words,
+                           (uint32_t)tid | (uint32_t)s << 8 | (uint32_t)(k - k0) << 11);
                 }
+            }
+        }
+        __syncthreads();   // queue complete
+        const int n = qn;
+        flushed = n >= kNT || k == k1 - 1;   // the same for every thread of the block
+        CubeGeo g;
+        for (int q = flushed ? tid : n; q < n; q += kNT) {
+            const uint32_t h = QH[q];
+            const int t = (int)(h & 255u);
+            g.Z0 = sz * (float)(k0 + (int)(h >> 11));
+            g.Z1 = g.Z0 + sz;
+            g.X0 = sx * (float)(x0 + t % kTX);
+            g.X1 = g.X0 + sx;
+            g.Y0 = sy * (float)(y0 + t / kTX);
+            g.Y1 = g.Y0 + sy;
+            float f[8];
+            if constexpr (MODE == 0) {
+#pragma unroll
+                for (int m = 0; m < 8; ++m) f[m] = (float)((QC[m >> 1][q] >> (16 * (m & 1))) & 0xFFFFu) / 2040.0f;
+            } else {
+                const uint32_t w = QC[0][q];
+#pragma unroll
+                for (int m = 0; m < 8; ++m) f[m] = (float)((w >> (4 * m)) & 15u) / 8.0f;
+            }
+            const double a = cube_area(f, g, iso, EV + tid);
+            if constexpr (MODE == 0) {
+                acc[0].add2(a, 0.0);
+            } else {
+                const unsigned face = (h >> 8) & 7u;
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    if (face == (unsigned)s) acc[s].add2(a, 0.0);
             }
         }
         cur ^= 1;
     }
 
     // Fixed-order block reduction.
-    const int lane = tid & 63, w = tid >> 6;
+    const int w = tid >> 6;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         DdSum a = acc[s];
