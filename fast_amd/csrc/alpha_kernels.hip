@@ -12,8 +12,9 @@
 // into the volume), byte/255: (sum of the 8 bytes) / 2040, which for a 0/255 volume is k/8
 // exactly.  Here those corner values are built from voxel sums instead of texture reads:
 // a block keeps, for its 64x4 tile and the plane being marched, the 2x2 in-plane voxel sums
-// of every lattice point (LDS), adds two consecutive planes of them into corner counts, and
-// each thread turns the 8 corner counts of its cube into the cube's triangles and area.
+// of every lattice point (LDS) and adds two consecutive planes of them into corner counts;
+// each thread classifies its cube from the 8 counts, and the cubes that cross the level set
+// are queued and triangulated by the whole block.
 //
 // Two modes share the march.  MODE 0 reads the volume's bytes (one surface).  MODE 1 reads
 // a byte of face bits per voxel (bit f: a pore voxel whose component touches face f; the

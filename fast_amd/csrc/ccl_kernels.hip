@@ -55,36 +55,56 @@ __device__ __forceinline__ void merge(uint32_t* P, uint32_t a, uint32_t b) {
     }
 }
 
+// Span starts, one wave per x-row in chunks of 64 voxels.  A voxel is in a span when the
+// number of value changes up to it (from `background` before the row) is odd, and its span
+// starts at the last change at or before it; both come from a ballot of the change flags,
+// with the parity and the last change carried from chunk to chunk.
 __global__ __launch_bounds__(256) void k_spans(const uint8_t* __restrict__ mask, uint32_t* __restrict__ P, int nx,
                                                int rows, uint8_t background) {
-    const int row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= rows) return;
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= rows) return;   // the whole wave
     const long long base = (long long)row * nx;
-    uint8_t prev = background;
-    bool open = false;
-    uint32_t start = kNone;
-    for (int x = 0; x < nx; ++x) {
-        const uint8_t v = mask[base + x];
-        if (v != prev) {
-            open = !open;
-            if (open) start = (uint32_t)(base + x);
-        }
-        P[base + x] = open ? start : kNone;
-        prev = v;
+    int prev_last = background;   // value before this chunk
+    unsigned parity = 0;          // number of changes before this chunk, mod 2
+    uint32_t last_change = kNone; // index of the last change before this chunk
+    const unsigned long long upto = (2ull << lane) - 1ull;   // lanes 0..lane
+    for (int x0 = 0; x0 < nx; x0 += 64) {
+        const int x = x0 + lane;
+        const bool in = x < nx;
+        const int v = in ? (int)mask[base + x] : 0;
+        int pv = __shfl_up(v, 1, 64);
+        if (lane == 0) pv = prev_last;
+        const unsigned long long changes = __ballot(in && v != pv);
+        const unsigned long long mine = changes & upto;
+        const bool open = ((parity + (unsigned)__popcll(mine)) & 1u) != 0u;
+        const uint32_t start = mine ? (uint32_t)(base + x0 + (63 - __clzll((long long)mine))) : last_change;
+        if (in) P[base + x] = open ? start : kNone;
+        parity = (parity + (unsigned)__popcll(changes)) & 1u;
+        if (changes) last_change = (uint32_t)(base + x0 + (63 - __clzll((long long)changes)));
+        prev_last = __shfl(v, 63, 64);
     }
 }
 
-__global__ __launch_bounds__(256) void k_union(uint32_t* P, int nx, int ny, int nz) {
+// Link every span to the spans it overlaps in the rows at y-1 and z-1.  A voxel whose byte
+// equals its x-1 neighbour's lies in the same span as that neighbour (spans change only where
+// the value changes), so where neither this row nor the other row changes value between x-1
+// and x, the pair of spans was already linked at x-1 and the voxel is skipped.  Span starts are
+// the only entries the merges write; every other in-span voxel keeps pointing at its start.
+__global__ __launch_bounds__(256) void k_union(const uint8_t* __restrict__ mask, uint32_t* P, int nx, int ny, int nz) {
     const long long n = (long long)nx * ny * nz;
     const long long plane = (long long)nx * ny;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         if (ld(P + i) == kNone) continue;
         const unsigned row = (unsigned)i / (unsigned)nx;   // labels are 32-bit, so i < 2^32
+        const int x = (int)((unsigned)i - row * (unsigned)nx);
         const int y = (int)(row % (unsigned)ny);
         const int z = (int)(row / (unsigned)ny);
-        // one link per span and row pair is enough, but linking every voxel keeps this simple
-        if (y > 0 && ld(P + i - nx) != kNone) merge(P, (uint32_t)i, (uint32_t)(i - nx));
-        if (z > 0 && ld(P + i - plane) != kNone) merge(P, (uint32_t)i, (uint32_t)(i - plane));
+        const bool same_here = x > 0 && mask[i] == mask[i - 1];
+        if (y > 0 && ld(P + i - nx) != kNone && !(same_here && mask[i - nx] == mask[i - nx - 1]))
+            merge(P, (uint32_t)i, (uint32_t)(i - nx));
+        if (z > 0 && ld(P + i - plane) != kNone && !(same_here && mask[i - plane] == mask[i - plane - 1]))
+            merge(P, (uint32_t)i, (uint32_t)(i - plane));
     }
 }
 
@@ -257,8 +277,8 @@ hipError_t ccl_label(const uint8_t* mask, int nx, int ny, int nz, uint8_t backgr
                      uint32_t* blocksum, uint32_t* num_roots, hipStream_t s) {
     const long long n = (long long)nx * ny * nz;
     const int rows = ny * nz;
-    hipLaunchKernelGGL(k_spans, dim3((rows + 255) / 256), dim3(256), 0, s, mask, P, nx, rows, background);
-    hipLaunchKernelGGL(k_union, dim3(grid_for(n)), dim3(256), 0, s, P, nx, ny, nz);
+    hipLaunchKernelGGL(k_spans, dim3((rows + 3) / 4), dim3(256), 0, s, mask, P, nx, rows, background);
+    hipLaunchKernelGGL(k_union, dim3(grid_for(n)), dim3(256), 0, s, mask, P, nx, ny, nz);
     hipLaunchKernelGGL(k_flatten, dim3(grid_for(n)), dim3(256), 0, s, P, n);
     const int nb = (int)((n + kScanChunk - 1) / kScanChunk);
     hipLaunchKernelGGL(k_count_roots, dim3(nb), dim3(kScanThreads), 0, s, (const uint32_t*)P, n, blocksum);

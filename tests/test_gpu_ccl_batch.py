@@ -42,6 +42,22 @@ def test_ccl_and_reachable_match_oracle(ctx, shape, frac, seed):
         assert np.array_equal(out, oout), d
 
 
+@pytest.mark.parametrize("shape,seed", [((40, 36, 70), 4), ((9, 130, 66), 5)])
+def test_ccl_multivalued_bytes_match_oracle(ctx, shape, seed):
+    """Bytes 0, 100 and 255 in runs: every change of value toggles a span, so equal
+    neighbouring bytes are the only sign of a continuing span (the union pass skips links
+    already made one voxel earlier on that sign)."""
+    rng = np.random.default_rng(seed)
+    runs = rng.choice(np.array([0, 100, 255], np.uint8), size=shape)
+    m = np.repeat(runs, 3, axis=2)[:, :, :shape[2]]
+    vol = ctx.volume(m)
+    lab, nl = vol.ccl_labels(255)
+    olab, onl = O.ccl(m)
+    assert nl == onl and np.array_equal(lab, olab)
+    assert np.array_equal(vol.area_tensor(), O.mc_area_tensor(m))
+    vol.close()
+
+
 def test_batch_equals_solo_bit_exact(ctx):
     rng = np.random.default_rng(5)
     vols, params = [], []
