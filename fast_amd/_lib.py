@@ -81,6 +81,17 @@ class ServerConfig(C.Structure):
 
 
 # (name, restype, argtypes) for every entry point of include/fast_tau.h
+class SegmentInfo(C.Structure):
+    """fast_segment_info: getCCLSegmentInfo's CCLSegmentInfo (VolumeSegmentation.h:58-64)."""
+    _fields_ = [
+        ("min_bb", C.c_int32 * 3),
+        ("max_bb", C.c_int32 * 3),
+        ("voxel_num", C.c_int64),
+        ("at_boundary", C.c_int32),
+        ("label_id", C.c_int32),
+    ]
+
+
 _u8p = C.POINTER(C.c_uint8)
 _u32p = C.POINTER(C.c_uint32)
 _dp = C.POINTER(C.c_double)
@@ -105,6 +116,7 @@ PROTOTYPES = [
     ("fast_ccl_labels", C.c_int, [_vp, C.c_uint8, _u32p, _u32p]),
     ("fast_reachable_mask", C.c_int, [_vp, C.c_int, C.c_uint8, _u8p, _u32p]),
     ("fast_ccl_boundary_table", C.c_int, [_vp, C.c_uint8, _u8p, C.c_uint32]),
+    ("fast_ccl_segment_info", C.c_int, [_vp, C.c_uint8, C.POINTER(SegmentInfo), C.c_uint32]),
     ("fast_alpha_area", C.c_int, [_vp, C.c_float, C.POINTER(C.c_double)]),
     ("fast_alpha_area_tensor", C.c_int, [_vp, C.c_float, C.POINTER(C.c_double)]),
     ("fast_debug_time_alpha", C.c_int, [_vp, C.c_int, C.POINTER(C.c_double)]),

@@ -12,6 +12,8 @@ namespace ftc {
 hipError_t ccl_label(const uint8_t* mask, int nx, int ny, int nz, uint8_t background, uint32_t* P, uint32_t* L,
                      uint32_t* blocksum, uint32_t* num_roots, hipStream_t s);
 hipError_t ccl_face_table(const uint32_t* L, int nx, int ny, int nz, uint8_t* table, uint32_t numLabels, hipStream_t s);
+hipError_t ccl_segment_stats(const uint32_t* L, int nx, int ny, int nz, uint32_t numLabels, uint32_t* info,
+                             hipStream_t s);
 hipError_t ccl_reachable(const uint32_t* L, long long n, const uint8_t* table, uint32_t numLabels, int dir, uint8_t* out,
                          hipStream_t s);
 }  // namespace ftc
@@ -102,6 +104,49 @@ int fast_ccl_boundary_table(fast_volume* v, uint8_t background, uint8_t* table, 
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     dfree(ctx, t);
     return hip_ok(ctx, e, "table readback") ? FAST_OK : FAST_E_HIP;
+}
+
+int fast_ccl_segment_info(fast_volume* v, uint8_t background, fast_segment_info* out, uint32_t num_labels) {
+    if (!v || !out) return FAST_E_BADARG;
+    fast_ctx* ctx = v->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    uint32_t nl = 0;
+    int rc = ensure_labels(v, background, &nl);
+    if (rc) return rc;
+    if (num_labels != nl) {
+        set_error(ctx, "segment info has %u labels, caller sized it for %u", nl, num_labels);
+        return FAST_E_BADARG;
+    }
+    uint32_t* d = (uint32_t*)dalloc(ctx, (size_t)7 * nl * sizeof(uint32_t));
+    if (!d) return FAST_E_OOM;
+    std::vector<uint32_t> h;
+    try {
+        h.resize((size_t)7 * nl);
+    } catch (const std::bad_alloc&) {
+        dfree(ctx, d);
+        return FAST_E_OOM;
+    }
+    hipError_t e = ftc::ccl_segment_stats(v->d_labels, v->nx, v->ny, v->nz, nl, d, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d, h.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    dfree(ctx, d);
+    if (!hip_ok(ctx, e, "segment info")) return FAST_E_HIP;
+    const int dim[3] = {v->nx, v->ny, v->nz};
+    for (uint32_t l = 0; l < nl; ++l) {
+        const uint32_t* r = &h[(size_t)7 * l];
+        fast_segment_info& o = out[l];
+        o.voxel_num = (int64_t)r[0];
+        bool at = false;
+        for (int a = 0; a < 3; ++a) {
+            o.min_bb[a] = (int32_t)r[1 + a];
+            o.max_bb[a] = (int32_t)r[4 + a] + 1;   // non-inclusive, as the reference
+            at = at || o.max_bb[a] == dim[a] || o.min_bb[a] == 0;
+        }
+        o.at_boundary = at ? 1 : 0;
+        o.label_id = (int32_t)l;
+    }
+    return FAST_OK;
 }
 
 int fast_reachable_mask(fast_volume* v, int dir, uint8_t background, uint8_t* out, uint32_t* num_labels) {

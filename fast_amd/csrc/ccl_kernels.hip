@@ -264,6 +264,44 @@ __global__ __launch_bounds__(256) void k_face_codes(const uint32_t* L, long long
     }
 }
 
+// Per-label statistics (getCCLSegmentInfo, VolumeSegmentation.cpp:135-178): voxel count and
+// bounding box.  info[7 l ..]: count, min x, y, z, max x, y, z (inclusive).  Every x-run of
+// equal labels contributes once, from its first voxel, with integer atomics (the result does
+// not depend on their order).
+__global__ __launch_bounds__(256) void k_seg_init(uint32_t* info, uint32_t numLabels, int nx, int ny, int nz) {
+    for (long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x; l < numLabels;
+         l += (long long)gridDim.x * blockDim.x) {
+        uint32_t* r = info + 7 * l;
+        r[0] = 0;
+        r[1] = (uint32_t)nx;
+        r[2] = (uint32_t)ny;
+        r[3] = (uint32_t)nz;
+        r[4] = r[5] = r[6] = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_seg_runs(const uint32_t* L, int nx, int ny, int nz, uint32_t* info) {
+    const long long n = (long long)nx * ny * nz;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const uint32_t l = L[i];
+        const unsigned row = (unsigned)i / (unsigned)nx;
+        const int x = (int)((unsigned)i - row * (unsigned)nx);
+        if (x > 0 && L[i - 1] == l) continue;   // not the first voxel of its run
+        int e = x;
+        while (e + 1 < nx && L[i + (e + 1 - x)] == l) ++e;
+        const int y = (int)(row % (unsigned)ny);
+        const int z = (int)(row / (unsigned)ny);
+        uint32_t* r = info + 7 * (size_t)l;
+        atomicAdd(r + 0, (uint32_t)(e - x + 1));
+        atomicMin(r + 1, (uint32_t)x);
+        atomicMin(r + 2, (uint32_t)y);
+        atomicMin(r + 3, (uint32_t)z);
+        atomicMax(r + 4, (uint32_t)e);
+        atomicMax(r + 5, (uint32_t)y);
+        atomicMax(r + 6, (uint32_t)z);
+    }
+}
+
 inline unsigned grid_for(long long n) {
     long long b = (n + 255) / 256;
     return (unsigned)(b > 65536 ? 65536 : (b < 1 ? 1 : b));
@@ -301,6 +339,14 @@ hipError_t ccl_reachable(const uint32_t* L, long long n, const uint8_t* table, u
                          hipStream_t s) {
     const int k = dir / 2;
     hipLaunchKernelGGL(k_reachable, dim3(grid_for(n)), dim3(256), 0, s, L, n, table, numLabels, 2 * k, 2 * k + 1, out);
+    return hipGetLastError();
+}
+
+hipError_t ccl_segment_stats(const uint32_t* L, int nx, int ny, int nz, uint32_t numLabels, uint32_t* info,
+                             hipStream_t s) {
+    const long long n = (long long)nx * ny * nz;
+    hipLaunchKernelGGL(k_seg_init, dim3(grid_for(numLabels)), dim3(256), 0, s, info, numLabels, nx, ny, nz);
+    hipLaunchKernelGGL(k_seg_runs, dim3(grid_for(n)), dim3(256), 0, s, L, nx, ny, nz, info);
     return hipGetLastError();
 }
 

@@ -14,7 +14,8 @@
 // `fast alpha` writes the reference's rows (AlphaModule::getAlpha, AlphaModule.h:28-174):
 // per direction the boundary-connected area of that face, then a DIR_NONE row with the
 // whole volume's area; with --basic, the whole volume's area on every direction's row.
-// Its --mesh (OBJ export) and --segmentinfo (segment CSV) outputs are not offered.
+// --segmentinfo FILE adds segmentVolumeRatio to the rows and writes the segments that do not
+// reach every face to FILE (AlphaModule.h:36-43,65-123).  --mesh (OBJ export) is not offered.
 #include "fast_tau.h"
 
 #include <algorithm>
@@ -147,13 +148,33 @@ void usage() {
                  "  --randsub X|X,Y,Z    subvolume of this size at a random origin\n"
                  "  --aspect X|X,Y,Z     aspect-ratio crop; one component must be 1.0\n"
                  "alpha:\n"
-                 "  --basic              whole-volume area only (no connectivity)\n";
+                 "  --basic              whole-volume area only (no connectivity)\n"
+                 "  --segmentinfo FILE   segments that do not reach every face, as CSV; adds\n"
+                 "                       segmentVolumeRatio to the rows\n";
 }
 
 // AlphaModule::getAlpha (AlphaModule.h:28-174): porosity as getPorosity<float>, the
 // per-face tensor (CCL included in its time, a sixth per row), then the DIR_NONE row.
+// filterSegmentInfo (VolumeSegmentation.cpp:182-205) with the inverted mask: labels >= 1 not
+// on row `row` of the per-face table (row 6: on every face).
+std::vector<const fast_segment_info*> filter_segments(const std::vector<fast_segment_info>& sinfo,
+                                                      const std::vector<uint8_t>& table, int row) {
+    const size_t nl = sinfo.size();
+    std::vector<const fast_segment_info*> out;
+    for (size_t l = 1; l < nl; ++l)
+        if (!table[(size_t)row * nl + l]) out.push_back(&sinfo[l]);
+    return out;
+}
+
+// getCCLSegmentRatio (VolumeSegmentation.cpp:207-215)
+float segment_ratio(const std::vector<const fast_segment_info*>& f, size_t n) {
+    int vox = 0;
+    for (auto* s : f) vox += (int)s->voxel_num;
+    return float(vox) / (float)(int)n;
+}
+
 int run_alpha(fast_ctx* ctx, fast_volume* vol, const std::string& input, const int (&dim)[3], uint64_t zeros,
-              bool basic, const std::vector<int>& dirs, const std::string& output) {
+              bool basic, const std::vector<int>& dirs, const std::string& output, const std::string& seginfo) {
     CsvOut out(output);
     Row o;
     o["ID"] = input;
@@ -175,10 +196,27 @@ int run_alpha(fast_ctx* ctx, fast_volume* vol, const std::string& input, const i
             std::cerr << "alpha failed: " << fast_ctx_last_error(ctx) << std::endl;
             return 1;
         }
+        std::vector<fast_segment_info> sinfo;
+        std::vector<uint8_t> table;
+        if (!seginfo.empty()) {
+            uint32_t nl = 0;
+            if (fast_ccl_labels(vol, 255, nullptr, &nl) != FAST_OK) {
+                std::cerr << "segment info failed: " << fast_ctx_last_error(ctx) << std::endl;
+                return 1;
+            }
+            sinfo.resize(nl);
+            table.resize((size_t)7 * nl);
+            if (fast_ccl_segment_info(vol, 255, sinfo.data(), nl) != FAST_OK ||
+                fast_ccl_boundary_table(vol, 255, table.data(), nl) != FAST_OK) {
+                std::cerr << "segment info failed: " << fast_ctx_last_error(ctx) << std::endl;
+                return 1;
+            }
+        }
         for (int d : dirs) {
             o["rad"] = fmt(tensor[d]);
             o["radTime"] = fmt(dt / 6.0f);
             o["dir"] = kDirNames[d];
+            if (!sinfo.empty()) o["segmentVolumeRatio"] = fmt(segment_ratio(filter_segments(sinfo, table, d), n));
             out.add(o);
         }
         auto [rc2, dt2] = timed([&] { return fast_alpha_area(vol, 0.5f, &whole); });
@@ -189,6 +227,25 @@ int run_alpha(fast_ctx* ctx, fast_volume* vol, const std::string& input, const i
         o["rad"] = fmt((float)whole);
         o["radTime"] = fmt(dt2);
         o["dir"] = "DIR_NONE";
+        if (!sinfo.empty()) {
+            const auto none = filter_segments(sinfo, table, 6);
+            o["segmentVolumeRatio"] = fmt(segment_ratio(none, n));
+            CsvOut info(seginfo);
+            for (auto* si : none) {
+                Row r;
+                r["ID"] = input;
+                r["minX"] = fmt(si->min_bb[0]);
+                r["minY"] = fmt(si->min_bb[1]);
+                r["minZ"] = fmt(si->min_bb[2]);
+                r["maxX"] = fmt(si->max_bb[0]);
+                r["maxY"] = fmt(si->max_bb[1]);
+                r["maxZ"] = fmt(si->max_bb[2]);
+                r["labelID"] = fmt(si->label_id);
+                r["voxelNum"] = fmt((int)si->voxel_num);
+                r["atBoundary"] = fmt(si->at_boundary);
+                info.add(r);
+            }
+        }
         out.add(o);
         return 0;
     }
@@ -223,7 +280,7 @@ int main(int argc, char** argv) {
         usage();
         return 1;
     }
-    std::string input, dirs = "x-", solver = "CG", prec = "double", output, reach;
+    std::string input, dirs = "x-", solver = "CG", prec = "double", output, reach, seginfo;
     int tol = 6, gpu = 0;
     long iters = 10000;
     bool verbose = false;
@@ -264,7 +321,9 @@ int main(int argc, char** argv) {
             return 1;
         } else if (alpha && a == "--basic") {
             basic = true;
-        } else if (alpha && (a == "--mesh" || a == "--segmentinfo")) {
+        } else if (alpha && a == "--segmentinfo") {
+            seginfo = next("--segmentinfo");
+        } else if (alpha && a == "--mesh") {
             std::cerr << a << ": not offered by this build" << std::endl;
             return 1;
         } else if (a == "--reach") {
@@ -399,7 +458,7 @@ int main(int argc, char** argv) {
     uint64_t zeros = 0;
     fast_volume_count_zero(vol, &zeros);
     if (alpha) {
-        const int status = run_alpha(ctx, vol, input, dim, zeros, basic, get_dirs(dirs), output);
+        const int status = run_alpha(ctx, vol, input, dim, zeros, basic, get_dirs(dirs), output, seginfo);
         fast_volume_destroy(vol);
         fast_ctx_destroy(ctx);
         return status;

@@ -207,6 +207,17 @@ class Volume:
               self.ctx.handle)
         return t
 
+    def segment_info(self, background: int = 255) -> list:
+        """Per-label statistics of the CCL labels (getCCLSegmentInfo,
+        VolumeSegmentation.cpp:135-178): a dict per label (label 0 included) with minBB
+        (x, y, z), maxBB (exclusive), voxelNum, atBoundary and labelID."""
+        from ._lib import SegmentInfo
+        _, nl = self.ccl_labels(background)
+        arr = (SegmentInfo * nl)()
+        check(lib().fast_ccl_segment_info(self._h, background, arr, nl), "fast_ccl_segment_info", self.ctx.handle)
+        return [{"minBB": tuple(r.min_bb), "maxBB": tuple(r.max_bb), "voxelNum": int(r.voxel_num),
+                 "atBoundary": bool(r.at_boundary), "labelID": int(r.label_id)} for r in arr]
+
     def reachable_mask(self, d: int, background: int = 255):
         out = np.empty(self.shape, dtype=np.uint8)
         nl = C.c_uint32()
@@ -342,6 +353,22 @@ def getReactiveAreaDensityTensor(mask, isovalue: float = 0.5, ctx: Optional[Cont
     finally:
         if vol is not mask:
             vol.close()
+
+
+def filterSegmentInfo(sinfo: list, table: np.ndarray, d: int = 6, invertMask: bool = True) -> list:
+    """filterSegmentInfo (VolumeSegmentation.cpp:182-205): the segments whose label is on row
+    `d` of the per-face table (Dir order, 6 = on every face), or with `invertMask` the labels
+    >= 1 that are not."""
+    row = np.asarray(table)[d].astype(bool)
+    keep = row.copy()
+    if invertMask:
+        keep[1:] = ~row[1:]
+    return [s for s, k in zip(sinfo, keep) if k]
+
+
+def getCCLSegmentRatio(sinfo: list, n_voxels: int) -> float:
+    """getCCLSegmentRatio (VolumeSegmentation.cpp:207-215): float(sum of voxelNum) / N."""
+    return float(np.float32(sum(s["voxelNum"] for s in sinfo)) / np.float32(n_voxels))
 
 
 def solve_batch(ctx: Context, vols: Sequence[Volume], params: Sequence[TauParams]) -> list:

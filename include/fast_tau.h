@@ -171,6 +171,19 @@ int fast_reachable_mask(fast_volume* vol, int dir, uint8_t background, uint8_t* 
 /* Per-face label table (VolumeCCL_BoundaryLabels, VolumeCCL.cu:753-818): 7 rows of
  * num_labels bytes (rows in Dir order, row 6 = present on all faces).  table: 7*num_labels. */
 int fast_ccl_boundary_table(fast_volume* vol, uint8_t background, uint8_t* table, uint32_t table_labels);
+/* Per-label segment statistics (getCCLSegmentInfo, VolumeSegmentation.cpp:135-178) of the
+ * labels for `background`: out[l] for every label l (0, the background, included) holds the
+ * voxel count, the bounding box (min inclusive, max exclusive: the reference's maxBB + 1) and
+ * whether the box touches a face of the volume.  out: num_labels entries (num_labels from
+ * fast_ccl_labels; a different count is refused). */
+typedef struct fast_segment_info {
+    int32_t min_bb[3];    /* x, y, z; the volume's dims for a label with no voxels */
+    int32_t max_bb[3];    /* exclusive */
+    int64_t voxel_num;
+    int32_t at_boundary;  /* max_bb == dim or min_bb == 0 on some axis */
+    int32_t label_id;
+} fast_segment_info;
+int fast_ccl_segment_info(fast_volume* vol, uint8_t background, fast_segment_info* out, uint32_t num_labels);
 
 /* ---- reactive area density (`fast alpha`) ---------------------------------- */
 /* Marching-cubes area of the `isovalue` level set (0.5 in the reference callers) of the

@@ -58,6 +58,48 @@ def test_ccl_multivalued_bytes_match_oracle(ctx, shape, seed):
     vol.close()
 
 
+def _segments_from_labels(lab, nl):
+    """getCCLSegmentInfo from the oracle's labels with numpy scatter-reductions."""
+    nz, ny, nx = lab.shape
+    flat = lab.ravel().astype(np.int64)
+    z, y, x = (a.ravel() for a in np.indices(lab.shape))
+    cnt = np.bincount(flat, minlength=nl)
+    mn = np.tile(np.array([nx, ny, nz]), (nl, 1))
+    mx = np.zeros((nl, 3), np.int64)
+    for a, c in enumerate((x, y, z)):
+        np.minimum.at(mn[:, a], flat, c)
+        np.maximum.at(mx[:, a], flat, c)
+    mx += 1
+    dim = np.array([nx, ny, nz])
+    at = (mx == dim).any(axis=1) | (mn == 0).any(axis=1)
+    return cnt, mn, mx, at
+
+
+@pytest.mark.parametrize("shape,frac,seed", [((40, 50, 70), 0.55, 7), ((1, 9, 300), 0.5, 8), ((64, 64, 64), None, 0)])
+def test_segment_info_matches_oracle_labels(ctx, shape, frac, seed):
+    if frac is None:
+        m = F.generate_spheres(64, 6.0, 0.60, 0)
+    else:
+        m = ((np.random.default_rng(seed).random(shape) < frac) * 255).astype(np.uint8)
+    vol = ctx.volume(m)
+    info = vol.segment_info(255)
+    olab, onl = O.ccl(m)
+    cnt, mn, mx, at = _segments_from_labels(olab, onl)
+    assert len(info) == onl
+    for l, r in enumerate(info):
+        assert r["labelID"] == l and r["voxelNum"] == cnt[l], l
+        assert r["minBB"] == tuple(mn[l]) and r["maxBB"] == tuple(mx[l]), l
+        assert r["atBoundary"] == bool(at[l]), l
+    # segments that reach no face / not every face, and their volume ratios
+    table = O.boundary_table(olab, onl)
+    for d in range(7):
+        f = F.filterSegmentInfo(info, table, d)
+        keep = [l for l in range(1, onl) if not table[d][l]]
+        assert [r["labelID"] for r in f] == keep
+        assert F.getCCLSegmentRatio(f, m.size) == float(np.float32(cnt[keep].sum()) / np.float32(m.size))
+    vol.close()
+
+
 def test_batch_equals_solo_bit_exact(ctx):
     rng = np.random.default_rng(5)
     vols, params = [], []

@@ -122,6 +122,24 @@ def test_cli_alpha_rows(tmp_path):
     assert float(rows[6]["rad"]) == _fmt6(float(np.float32(O.mc_area(m))))
     por = np.float32(np.count_nonzero(m == 0)) / np.float32(m.size)
     assert all(float(row["porosity"]) == _fmt6(float(por)) for row in rows)
+    # --segmentinfo: the ratio column and the segments that do not reach every face
+    out_s, info_csv = tmp_path / "seg.csv", tmp_path / "info.csv"
+    r = subprocess.run([str(EXE), "alpha", str(vin), "-d", "all", "-o", str(out_s), "--segmentinfo", str(info_csv)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rows = list(csv.DictReader(open(out_s)))
+    lab, nl = O.ccl(m)
+    table = O.boundary_table(lab, nl)
+    counts = np.bincount(lab.ravel().astype(np.int64), minlength=nl)
+    for k, row in enumerate(rows):
+        keep = [l for l in range(1, nl) if not table[k][l]]   # rows in Dir order, then DIR_NONE (6)
+        ratio = np.float32(counts[keep].sum()) / np.float32(m.size)
+        assert float(row["segmentVolumeRatio"]) == _fmt6(float(ratio)), row["dir"]
+    info = list(csv.DictReader(open(info_csv)))
+    keep = [l for l in range(1, nl) if not table[6][l]]
+    assert [int(i["labelID"]) for i in info] == keep
+    assert all(int(i["voxelNum"]) == counts[int(i["labelID"])] for i in info)
+    assert set(info[0]) == {"ID", "minX", "minY", "minZ", "maxX", "maxY", "maxZ", "labelID", "voxelNum", "atBoundary"}
     out_b = tmp_path / "basic.csv"
     r = subprocess.run([str(EXE), "alpha", str(vin), "--basic", "-d", "xy", "-o", str(out_b)], capture_output=True,
                        text=True, timeout=120)
