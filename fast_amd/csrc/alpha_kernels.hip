@@ -29,6 +29,7 @@
 // volume sum is double-double with one final rounding, so it does not depend on the
 // reduction tree.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <cstdint>
 
@@ -92,6 +93,26 @@ constexpr uint64_t kEdgeDesc =
     edge_desc(0, 0, 0, 0) << 0 | edge_desc(1, 1, 0, 0) << 5 | edge_desc(0, 0, 1, 0) << 10 | edge_desc(1, 0, 0, 0) << 15 |
     edge_desc(0, 0, 0, 1) << 20 | edge_desc(1, 1, 0, 1) << 25 | edge_desc(0, 0, 1, 1) << 30 | edge_desc(1, 0, 0, 1) << 35 |
     edge_desc(2, 0, 0, 0) << 40 | edge_desc(2, 1, 0, 0) << 45 | edge_desc(2, 1, 1, 0) << 50 | edge_desc(2, 0, 1, 0) << 55;
+
+// Vertex on edge e (corners a -> b as in the table's edge list), by cases: used where a
+// thread triangulates its own cube (the mesh export).
+__device__ __forceinline__ void edge_vertex_switch(unsigned e, const float (&f)[8], const CubeGeo& g, float iso,
+                                                   float& x, float& y, float& z) {
+    switch (e) {
+        case 0: x = lerp_axis(iso, f[0], f[1], g.X0, g.X1); y = g.Y0; z = g.Z0; break;
+        case 1: x = g.X1; y = lerp_axis(iso, f[1], f[2], g.Y0, g.Y1); z = g.Z0; break;
+        case 2: x = lerp_axis(iso, f[2], f[3], g.X1, g.X0); y = g.Y1; z = g.Z0; break;
+        case 3: x = g.X0; y = lerp_axis(iso, f[3], f[0], g.Y1, g.Y0); z = g.Z0; break;
+        case 4: x = lerp_axis(iso, f[4], f[5], g.X0, g.X1); y = g.Y0; z = g.Z1; break;
+        case 5: x = g.X1; y = lerp_axis(iso, f[5], f[6], g.Y0, g.Y1); z = g.Z1; break;
+        case 6: x = lerp_axis(iso, f[6], f[7], g.X1, g.X0); y = g.Y1; z = g.Z1; break;
+        case 7: x = g.X0; y = lerp_axis(iso, f[7], f[4], g.Y1, g.Y0); z = g.Z1; break;
+        case 8: x = g.X0; y = g.Y0; z = lerp_axis(iso, f[0], f[4], g.Z0, g.Z1); break;
+        case 9: x = g.X1; y = g.Y0; z = lerp_axis(iso, f[1], f[5], g.Z0, g.Z1); break;
+        case 10: x = g.X1; y = g.Y1; z = lerp_axis(iso, f[2], f[6], g.Z0, g.Z1); break;
+        default: x = g.X0; y = g.Y1; z = lerp_axis(iso, f[3], f[7], g.Z0, g.Z1); break;
+    }
+}
 
 // Marching-cubes area of one cube with corner values f (table order of the triangles).  The
 // interpolated coordinate of every edge is computed once, without branching on the edge, and
@@ -386,6 +407,146 @@ hipError_t mc_area(const uint8_t* src, int nx, int ny, int nz, float iso, int fa
         hipLaunchKernelGGL(k_mc_area<0>, grid, dim3(kNT), 0, s, src, nx, ny, nz, iso, thr, part);
         hipLaunchKernelGGL(k_mc_final, dim3(1), dim3(256), 0, s, part, nblk, 1, out);
     }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------
+// Triangle mesh for the OBJ export (`fast alpha --mesh`, AlphaModule.h:145-167): the volume
+// padded with one zero voxel per side (Volume::withZeroPadding), marching cubes over the
+// padded grid with output positions in a box centred at the origin, spacing 1/maxDim
+// (getVolumeAreaMesh, VolumeSurface.cpp:39-74; ___generateTriangles, VolumeSurface.cu:260-357).
+// Triangles are emitted in linear cube order, then table order, as the reference's compaction
+// by exclusive scan orders them.
+// ---------------------------------------------------------------------------------------
+namespace {
+
+__global__ __launch_bounds__(256) void k_pad(const uint8_t* __restrict__ src, int nx, int ny, int nz,
+                                             uint8_t* __restrict__ dst) {
+    const int px = nx + 2, py = ny + 2, pz = nz + 2;
+    const long long n = (long long)px * py * pz;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int x = (int)(i % px), y = (int)((i / px) % py), z = (int)(i / ((long long)px * py));
+        const bool inside = x >= 1 && x <= nx && y >= 1 && y <= ny && z >= 1 && z <= nz;
+        dst[i] = inside ? src[(size_t)(x - 1) + (size_t)nx * ((size_t)(y - 1) + (size_t)ny * (size_t)(z - 1))] : 0;
+    }
+}
+
+// Byte sums of the 8 clamped voxels around each corner of cube (x, y, z) of an r0 x r1 x r2 volume.
+__device__ __forceinline__ void corner_sums(const uint8_t* v, int r0, int r1, int r2, int x, int y, int z,
+                                            uint32_t (&c)[8]) {
+    const int dx[8] = {0, 1, 1, 0, 0, 1, 1, 0}, dy[8] = {0, 0, 1, 1, 0, 0, 1, 1}, dz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        uint32_t s = 0;
+#pragma unroll
+        for (int k = -1; k <= 0; ++k)
+#pragma unroll
+            for (int j = -1; j <= 0; ++j)
+#pragma unroll
+                for (int i = -1; i <= 0; ++i) {
+                    const int a = min(max(x + dx[m] + i, 0), r0 - 1);
+                    const int b = min(max(y + dy[m] + j, 0), r1 - 1);
+                    const int cc = min(max(z + dz[m] + k, 0), r2 - 1);
+                    s += v[(size_t)a + (size_t)r0 * ((size_t)b + (size_t)r1 * (size_t)cc)];
+                }
+        c[m] = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mesh_count(const uint8_t* __restrict__ v, int r0, int r1, int r2,
+                                                    uint32_t thr, uint32_t* __restrict__ count) {
+    const long long n = (long long)r0 * r1 * r2;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int x = (int)(i % r0), y = (int)((i / r0) % r1), z = (int)(i / ((long long)r0 * r1));
+        uint32_t c[8];
+        corner_sums(v, r0, r1, r2, x, y, z, c);
+        unsigned in = 0;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) in |= (c[m] < thr ? 1u : 0u) << m;
+        count[i] = 3u * (uint32_t)(c_tri[in] >> 60);
+    }
+}
+
+// out: 6 floats per vertex (position, then the triangle's normal), at 6 * offset[cube].
+__global__ __launch_bounds__(256) void k_mesh_emit(const uint8_t* __restrict__ v, int r0, int r1, int r2, float iso,
+                                                   const uint32_t* __restrict__ count,
+                                                   const uint32_t* __restrict__ offset, float* __restrict__ out) {
+    const long long n = (long long)r0 * r1 * r2;
+    const float tvs = 1.0f / (float)max(r0, max(r1, r2));   // params.dx = dy = dz = 1 / maxDim
+    const float hx = 0.5f * (tvs * (float)r0), hy = 0.5f * (tvs * (float)r1), hz = 0.5f * (tvs * (float)r2);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        if (count[i] == 0) continue;
+        const int x = (int)(i % r0), y = (int)((i / r0) % r1), z = (int)(i / ((long long)r0 * r1));
+        uint32_t c[8];
+        corner_sums(v, r0, r1, r2, x, y, z, c);
+        float f[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) f[m] = (float)c[m] / 2040.0f;
+        unsigned ci = 0;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) ci |= (f[m] < iso ? 1u : 0u) << m;
+        CubeGeo g;
+        g.X0 = tvs * (float)x - hx;
+        g.X1 = g.X0 + tvs;
+        g.Y0 = tvs * (float)y - hy;
+        g.Y1 = g.Y0 + tvs;
+        g.Z0 = tvs * (float)z - hz;
+        g.Z1 = g.Z0 + tvs;
+        const uint64_t w = c_tri[ci];
+        const int ntri = min((int)(w >> 60), (int)(count[i] / 3u));   // never beyond the space counted
+        float* o = out + 6 * (size_t)offset[i];
+        for (int t = 0; t < ntri; ++t) {
+            float p[3][3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+                edge_vertex_switch((unsigned)(w >> (4 * (3 * t + q))) & 15u, f, g, iso, p[q][0], p[q][1], p[q][2]);
+            const float ux = p[1][0] - p[0][0], uy = p[1][1] - p[0][1], uz = p[1][2] - p[0][2];
+            const float vx = p[2][0] - p[0][0], vy = p[2][1] - p[0][1], vz = p[2][2] - p[0][2];
+            const float nxv = uy * vz - uz * vy, nyv = uz * vx - ux * vz, nzv = ux * vy - uy * vx;
+            const float inv = 1.0f / sqrtf(nxv * nxv + nyv * nyv + nzv * nzv);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                float* d = o + 6 * (3 * t + q);
+                d[0] = p[q][0];
+                d[1] = p[q][1];
+                d[2] = p[q][2];
+                d[3] = nxv * inv;
+                d[4] = nyv * inv;
+                d[5] = nzv * inv;
+            }
+        }
+    }
+}
+
+inline unsigned mesh_grid(long long n) {
+    long long b = (n + 255) / 256;
+    return (unsigned)(b > 65536 ? 65536 : (b < 1 ? 1 : b));
+}
+
+}  // namespace
+
+// Padded volume and per-cube vertex counts.  padded: (nx+2)(ny+2)(nz+2) bytes; count: as many
+// uint32.  thr as for the area kernels.
+hipError_t mesh_count(const uint8_t* src, int nx, int ny, int nz, float iso, uint8_t* padded, uint32_t* count,
+                      hipStream_t s) {
+    const long long n = (long long)(nx + 2) * (ny + 2) * (nz + 2);
+    uint32_t thr = 0;
+    while (thr <= 2040u && (float)thr / 2040.0f < iso) ++thr;
+    hipLaunchKernelGGL(k_pad, dim3(mesh_grid(n)), dim3(256), 0, s, src, nx, ny, nz, padded);
+    hipLaunchKernelGGL(k_mesh_count, dim3(mesh_grid(n)), dim3(256), 0, s, padded, nx + 2, ny + 2, nz + 2, thr, count);
+    return hipGetLastError();
+}
+
+// Exclusive prefix sum of n per-cube counts (hipCUB).  Call with tmp = nullptr for *tmp_bytes.
+hipError_t mesh_scan(const uint32_t* count, uint32_t* offset, int n, void* tmp, size_t* tmp_bytes, hipStream_t s) {
+    return hipcub::DeviceScan::ExclusiveSum(tmp, *tmp_bytes, count, offset, n, s);
+}
+
+hipError_t mesh_emit(const uint8_t* padded, int nx, int ny, int nz, float iso, const uint32_t* count,
+                     const uint32_t* offset, float* out, hipStream_t s) {
+    const long long n = (long long)(nx + 2) * (ny + 2) * (nz + 2);
+    hipLaunchKernelGGL(k_mesh_emit, dim3(mesh_grid(n)), dim3(256), 0, s, padded, nx + 2, ny + 2, nz + 2, iso, count,
+                       offset, out);
     return hipGetLastError();
 }
 

@@ -15,7 +15,8 @@
 // per direction the boundary-connected area of that face, then a DIR_NONE row with the
 // whole volume's area; with --basic, the whole volume's area on every direction's row.
 // --segmentinfo FILE adds segmentVolumeRatio to the rows and writes the segments that do not
-// reach every face to FILE (AlphaModule.h:36-43,65-123).  --mesh (OBJ export) is not offered.
+// reach every face to FILE (AlphaModule.h:36-43,65-123).  --mesh FILE writes the triangle mesh
+// of the zero-padded volume as OBJ (AlphaModule.h:145-167).
 #include "fast_tau.h"
 
 #include <algorithm>
@@ -150,7 +151,8 @@ void usage() {
                  "alpha:\n"
                  "  --basic              whole-volume area only (no connectivity)\n"
                  "  --segmentinfo FILE   segments that do not reach every face, as CSV; adds\n"
-                 "                       segmentVolumeRatio to the rows\n";
+                 "                       segmentVolumeRatio to the rows\n"
+                 "  --mesh FILE          triangle mesh of the zero-padded volume as OBJ\n";
 }
 
 // AlphaModule::getAlpha (AlphaModule.h:28-174): porosity as getPorosity<float>, the
@@ -280,7 +282,7 @@ int main(int argc, char** argv) {
         usage();
         return 1;
     }
-    std::string input, dirs = "x-", solver = "CG", prec = "double", output, reach, seginfo;
+    std::string input, dirs = "x-", solver = "CG", prec = "double", output, reach, seginfo, mesh;
     int tol = 6, gpu = 0;
     long iters = 10000;
     bool verbose = false;
@@ -324,8 +326,7 @@ int main(int argc, char** argv) {
         } else if (alpha && a == "--segmentinfo") {
             seginfo = next("--segmentinfo");
         } else if (alpha && a == "--mesh") {
-            std::cerr << a << ": not offered by this build" << std::endl;
-            return 1;
+            mesh = next("--mesh");
         } else if (a == "--reach") {
             reach = next("--reach");
         } else if (a == "--gpu") {
@@ -458,7 +459,19 @@ int main(int argc, char** argv) {
     uint64_t zeros = 0;
     fast_volume_count_zero(vol, &zeros);
     if (alpha) {
-        const int status = run_alpha(ctx, vol, input, dim, zeros, basic, get_dirs(dirs), output, seginfo);
+        int status = run_alpha(ctx, vol, input, dim, zeros, basic, get_dirs(dirs), output, seginfo);
+        if (status == 0 && !mesh.empty()) {   // AlphaModule.h:145-167
+            size_t nv = 0;
+            if (fast_alpha_save_obj(vol, 0.5f, mesh.c_str(), &nv) != FAST_OK) {
+                std::cerr << "mesh export failed: " << fast_ctx_last_error(ctx) << std::endl;
+                status = 1;
+            } else if (nv > 0) {
+                std::cout << "Saving " << mesh << ", Vertices:" << nv << std::endl;
+                std::cout << "Done." << std::endl;
+            } else {
+                std::cerr << "Has no triangles" << std::endl;
+            }
+        }
         fast_volume_destroy(vol);
         fast_ctx_destroy(ctx);
         return status;

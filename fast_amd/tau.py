@@ -240,6 +240,26 @@ class Volume:
               "fast_alpha_area_tensor", self.ctx.handle)
         return out
 
+    def mesh(self, isovalue: float = 0.5) -> np.ndarray:
+        """Triangle mesh of the zero-padded volume as `fast alpha --mesh` builds it
+        (getVolumeAreaMesh, VolumeSurface.cpp:39-74): (n_vertices, 6) float32, position and
+        triangle normal, three vertices per triangle."""
+        n = C.c_size_t()
+        check(lib().fast_alpha_mesh(self._h, float(isovalue), None, 0, C.byref(n)), "fast_alpha_mesh", self.ctx.handle)
+        out = np.empty((n.value, 6), dtype=np.float32)
+        if n.value:
+            check(lib().fast_alpha_mesh(self._h, float(isovalue), out.ctypes.data_as(C.POINTER(C.c_float)), n.value,
+                                        C.byref(n)), "fast_alpha_mesh", self.ctx.handle)
+        return out
+
+    def save_obj(self, path: str, isovalue: float = 0.5) -> int:
+        """The mesh as the reference's OBJ (CUDA_VBO::saveObj, DataPtr.cpp:684-740); returns the
+        number of triangle vertices (0: empty mesh, no file written)."""
+        n = C.c_size_t()
+        check(lib().fast_alpha_save_obj(self._h, float(isovalue), str(path).encode(), C.byref(n)),
+              "fast_alpha_save_obj", self.ctx.handle)
+        return int(n.value)
+
     def apply(self, d: int, x: np.ndarray, precision="double") -> np.ndarray:
         """y = A x with the solver's matrix-free operator (diagnostic)."""
         if _as_precision(precision) == Precision.FP32:

@@ -201,6 +201,19 @@ int fast_alpha_area(fast_volume* vol, float isovalue, double* area);
  * above on the volume that is 255 on the pore voxels whose component touches face f and 0
  * elsewhere (generateBoundaryConnectedVolume, VolumeSegmentation.cpp:66-90).  area: 6. */
 int fast_alpha_area_tensor(fast_volume* vol, float isovalue, double area[6]);
+/* Triangle mesh of the volume padded with one zero voxel per side, as `fast alpha --mesh`
+ * builds it (AlphaModule.h:145-167: withZeroPadding(1, 1), getVolumeAreaMesh(padded, false);
+ * VolumeSurface.cpp:39-74, VolumeSurface.cu:260-505): marching cubes over the padded grid,
+ * positions in a box centred at the origin with spacing 1/maxDim of the padded dims, three
+ * vertices per triangle in cube order then table order, each carrying its triangle's unit
+ * normal.  *nverts = number of vertices; when verts is non-null and capacity (in vertices) is
+ * at least that, verts receives 6 floats per vertex (x, y, z, nx, ny, nz).  Volumes whose
+ * padded cube count times 15 reaches 2^32 are refused. */
+int fast_alpha_mesh(fast_volume* vol, float isovalue, float* verts, size_t capacity, size_t* nverts);
+/* The same mesh written as the reference's OBJ (CUDA_VBO::saveObj, DataPtr.cpp:684-740):
+ * vertices merged by equal position in order of first appearance, normals summed and
+ * normalised, `v` / `vn` / `f` lines.  No file is written for an empty mesh (*nverts = 0). */
+int fast_alpha_save_obj(fast_volume* vol, float isovalue, const char* path, size_t* nverts);
 
 /* ---- serving: continuous batching ------------------------------------------ */
 /* A request loop that admits tau solves of mixed sizes by HBM budget, keeps masks and

@@ -17,6 +17,10 @@
  *   - triangle area    VolumeSurface.cu:187-189, cross/dot/length CudaMath.h:1416,1244,1287
  *   - per-cube sum     VolumeSurface.cu:240-255 (double, table order)
  *   - volume sum       Volume::sum (order unspecified): here double-double, rounded once
+ *   - mesh (--mesh)    AlphaModule.h:145-167, getVolumeAreaMesh (VolumeSurface.cpp:39-74),
+ *                      ___generateTriangles (VolumeSurface.cu:260-357); the triangle normal
+ *                      is n / sqrt(n.n) where the reference uses CUDA's approximate rsqrtf,
+ *                      so normals agree with the reference to a few ulp only
  *   - tensor           getReactiveAreaDensityTensor, VolumeMeasures.cpp:303-333: face f's
  *                      volume is 255 on pore voxels whose label is on face f
  *                      (generateBoundaryConnectedVolume, VolumeSegmentation.cpp:66-90), from
@@ -31,6 +35,7 @@
  * gives the chamfered box of the formula in the test, per-face tensor values of split
  * volumes); the sphere area against 4 pi r^2.
  */
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -151,6 +156,65 @@ double oracle_mc_area(const uint8_t* vol, int nx, int ny, int nz, float iso) {
     DDs total;
     for (const DDs& p : part) total.add2(p.hi, p.lo);
     return total.hi + total.lo;
+}
+
+/* The `--mesh` triangles (AlphaModule.h:145-167; getVolumeAreaMesh, VolumeSurface.cpp:39-74;
+ * ___generateTriangles, VolumeSurface.cu:260-357): the volume padded with one zero voxel per
+ * side, cubes in linear order, output positions targetPos = (1/maxDim) * ivox - 0.5 * (1/maxDim)
+ * * res plus the corner offsets, three vertices per triangle with the triangle's normal
+ * normalize(cross(v1 - v0, v2 - v0)), here n * (1 / sqrt(n.n)).  Writes up to `cap` vertices
+ * (6 floats each) and returns the vertex count. */
+size_t oracle_mc_mesh(const uint8_t* vol, int nx, int ny, int nz, float iso, float* out, size_t cap) {
+    const int rx = nx + 2, ry = ny + 2, rz = nz + 2;
+    std::vector<uint8_t> pad((size_t)rx * ry * rz, 0);
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x)
+                pad[(size_t)(x + 1) + (size_t)rx * ((size_t)(y + 1) + (size_t)ry * (size_t)(z + 1))] =
+                    vol[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * (size_t)z)];
+    const int maxDim = std::max(rx, std::max(ry, rz));
+    const float tvs = 1.0f / (float)maxDim;
+    const F3 half = {0.5f * (tvs * (float)rx), 0.5f * (tvs * (float)ry), 0.5f * (tvs * (float)rz)};
+    size_t nv = 0;
+    for (int z = 0; z < rz; ++z)
+        for (int y = 0; y < ry; ++y)
+            for (int x = 0; x < rx; ++x) {
+                float f[8];
+                F3 p[8];
+                unsigned ci = 0;
+                const F3 tp = sub({tvs * (float)x, tvs * (float)y, tvs * (float)z}, half);
+                for (int m = 0; m < 8; ++m) {
+                    f[m] = corner_value(pad.data(), rx, ry, rz, x + kCorner[m][0], y + kCorner[m][1], z + kCorner[m][2]);
+                    p[m] = {tp.x + (kCorner[m][0] ? tvs : 0.0f), tp.y + (kCorner[m][1] ? tvs : 0.0f),
+                            tp.z + (kCorner[m][2] ? tvs : 0.0f)};
+                    if (f[m] < iso) ci |= 1u << m;
+                }
+                const uint64_t w = kTri[ci];
+                const int ntri = (int)(w >> 60);
+                for (int t = 0; t < ntri; ++t) {
+                    F3 v[3];
+                    for (int q = 0; q < 3; ++q) {
+                        const int e = (int)((w >> (4 * (3 * t + q))) & 15u);
+                        v[q] = vertex_interp(iso, p[kEdgeA[e]], p[kEdgeB[e]], f[kEdgeA[e]], f[kEdgeB[e]]);
+                    }
+                    const F3 a = sub(v[1], v[0]), b = sub(v[2], v[0]);
+                    const F3 c = {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+                    const float inv = 1.0f / std::sqrt(c.x * c.x + c.y * c.y + c.z * c.z);
+                    for (int q = 0; q < 3; ++q) {
+                        if (nv < cap) {
+                            float* d = out + 6 * nv;
+                            d[0] = v[q].x;
+                            d[1] = v[q].y;
+                            d[2] = v[q].z;
+                            d[3] = c.x * inv;
+                            d[4] = c.y * inv;
+                            d[5] = c.z * inv;
+                        }
+                        ++nv;
+                    }
+                }
+            }
+    return nv;
 }
 
 /* getReactiveAreaDensityTensor over getVolumeCCL(mask, 255): out[f] for the six faces. */

@@ -71,6 +71,8 @@ def lib():
         h.oracle_mc_cube_area.argtypes = [u8, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float]
         h.oracle_mc_area.restype = C.c_double
         h.oracle_mc_area.argtypes = [u8, C.c_int, C.c_int, C.c_int, C.c_float]
+        h.oracle_mc_mesh.restype = C.c_size_t
+        h.oracle_mc_mesh.argtypes = [u8, C.c_int, C.c_int, C.c_int, C.c_float, fp, C.c_size_t]
         h.oracle_mc_area_tensor.restype = None
         h.oracle_mc_area_tensor.argtypes = [u8, C.c_int, C.c_int, C.c_int, C.c_float, dp]
         _lib = h
@@ -199,4 +201,13 @@ def mc_area_tensor(mask, isovalue=0.5):
     m, p, nx, ny, nz = _m(mask)
     out = np.empty(6, dtype=np.float64)
     lib().oracle_mc_area_tensor(p, nx, ny, nz, float(isovalue), _dp(out))
+    return out
+
+
+def mc_mesh(mask, isovalue=0.5):
+    """The --mesh triangles: (n_vertices, 6) float32 array of position and triangle normal."""
+    m, p, nx, ny, nz = _m(mask)
+    n = lib().oracle_mc_mesh(p, nx, ny, nz, float(isovalue), None, 0)
+    out = np.empty((n, 6), dtype=np.float32)
+    lib().oracle_mc_mesh(p, nx, ny, nz, float(isovalue), out.ctypes.data_as(C.POINTER(C.c_float)), n)
     return out

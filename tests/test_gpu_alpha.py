@@ -111,3 +111,49 @@ def test_non_finite_isovalue_refused(ctx):
     with pytest.raises(FastError):
         vol.area_tensor(float("inf"))
     vol.close()
+
+
+MESH_SHAPES = [
+    ("box_in_pore", lambda: np.pad(np.full((3, 4, 5), 255, np.uint8), ((2, 3), (1, 2), (3, 1)))),
+    ("odd_19x23x37", lambda: rand_mask((19, 23, 37), 0.45, 3)),
+    ("thin_67x3x1", lambda: rand_mask((1, 3, 67), 0.5, 5)),
+    ("spheres_40", lambda: F.generate_spheres(40, 4.0, 0.5, 9)),
+]
+
+
+def _tri_areas(v):
+    p = v[:, :3].astype(np.float64).reshape(-1, 3, 3)
+    return np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0])
+
+
+@pytest.mark.parametrize("name,make", MESH_SHAPES, ids=[s[0] for s in MESH_SHAPES])
+def test_mesh_equals_the_oracle_and_is_closed(ctx, name, make):
+    """The --mesh triangles equal the oracle's bit for bit (NaN normals of degenerate triangles
+    included); the surface of the zero-padded volume is closed (vector area 0); for a cube the
+    triangles' total area equals the area kernel's on the padded volume."""
+    m = make()
+    vol = ctx.volume(m)
+    try:
+        got = vol.mesh()
+        ref = O.mc_mesh(m)
+        assert got.shape == ref.shape and got.shape[0] % 3 == 0, (name, got.shape, ref.shape)
+        assert np.array_equal(got, ref, equal_nan=True), name
+        cr = _tri_areas(got)
+        total = 0.5 * np.linalg.norm(cr, axis=1).sum()
+        assert total > 0
+        assert np.abs(cr.sum(axis=0)).max() <= 1e-5 * total, name
+        if len(set(m.shape)) == 1:
+            # same triangles; the mesh's coordinates are centred (|c| up to 0.5), so each one
+            # carries a rounding of up to 3e-8, a few 1e-6 of a voxel edge at 40^3
+            padded = np.pad(m, 1)
+            pvol = ctx.volume(padded)
+            assert total == pytest.approx(pvol.area(), rel=1e-5)
+            pvol.close()
+    finally:
+        vol.close()
+
+
+def test_mesh_of_empty_volume(ctx):
+    vol = ctx.volume(np.zeros((5, 6, 7), np.uint8))
+    assert vol.mesh().shape == (0, 6)
+    vol.close()

@@ -147,3 +147,31 @@ def test_cli_alpha_rows(tmp_path):
     rows = list(csv.DictReader(open(out_b)))
     assert len(rows) == 2 and "dir" not in rows[0]
     assert all(float(row["radBasic"]) == _fmt6(float(np.float32(O.mc_area(m)))) for row in rows)
+
+
+def test_cli_alpha_mesh_obj(tmp_path):
+    """`fast alpha --mesh`: the OBJ holds the oracle's triangles with vertices merged by equal
+    position in order of first appearance, one `vn` per vertex and faces in triangle order."""
+    m = np.pad(F.generate_spheres(24, 3.0, 0.4, 5), ((0, 2), (1, 0), (0, 0)))
+    vin, obj = tmp_path / "v.bin", tmp_path / "m.obj"
+    F.save_bin(str(vin), m)
+    r = subprocess.run([str(EXE), "alpha", str(vin), "--mesh", str(obj), "-o", str(tmp_path / "a.csv")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    ref = O.mc_mesh(m)
+    assert f"Vertices:{len(ref)}" in r.stdout
+    uniq, first = {}, []
+    for p in map(tuple, ref[:, :3].tolist()):
+        if p not in uniq:
+            uniq[p] = len(uniq)
+            first.append(p)
+    vs, vns, fs = [], [], []
+    for line in open(obj):
+        tag, *vals = line.split()
+        (vs if tag == "v" else vns if tag == "vn" else fs).append(vals)
+    assert len(vs) == len(uniq) == len(vns) and len(fs) == len(ref) // 3
+    for p, line in zip(first, vs):
+        assert [float(a) for a in line] == [_fmt6(c) for c in p]
+    ids = [uniq[p] + 1 for p in map(tuple, ref[:, :3].tolist())]
+    assert [int(a) for f in fs for a in f] == ids
+
