@@ -107,7 +107,10 @@ typedef struct fast_ctx fast_ctx;
  * FAST_TAU_GROUP_MB (default 320) is the red-black state size of the consecutive sub-batches a
  * batch of large volumes runs in (0 = always one launch; results are the same either way);
  * FAST_TAU_UPDATE_EVERY (default 4) is the number of mixed-form iterations between launches of
- * the reliable-update step (results are the same for every value). */
+ * the reliable-update step (results are the same for every value); FAST_TAU_RB_TZ forces the
+ * z depth of the red-black tiles (default: nz * tiles per plane / 2048, clamped to 4..32);
+ * FAST_TAU_RB_STAGED=0 selects the red-black Q pass that reads its Y neighbours from memory
+ * instead of staging them in LDS (results are bit-identical). */
 int fast_ctx_create(int device, size_t hbm_budget, fast_ctx** out);
 void fast_ctx_destroy(fast_ctx* ctx);
 /* Last error message of this context (never NULL). */
