@@ -181,7 +181,7 @@ __global__ __launch_bounds__(kNT) void k_mc_area(const uint8_t* __restrict__ src
     __shared__ double red[kNT / 64][NS][2];
     __shared__ float EV[12 * kNT];            // per-thread edge coordinates of the cube being triangulated
     __shared__ uint32_t QC[QW][KQ];           // queued corner counts (MODE 0: 2 x 16 bits, MODE 1: 8 nibbles)
-    __shared__ uint32_t QH[KQ];               // queued cube: thread of the tile | surface << 8 | layer << 11
+    __shared__ uint32_t QH[KQ];               // queued cube: thread of the tile | surface mask << 8 | layer << 14
     __shared__ int qn;
 
     const int tid = threadIdx.x;
@@ -272,25 +272,41 @@ __global__ __launch_bounds__(kNT) void k_mc_area(const uint8_t* __restrict__ src
 #pragma unroll
             for (int m = 0; m < 8; ++m) in |= (c[m] < thr ? 1u : 0u) << m;
             const uint32_t words[QW] = {c[0] | c[1] << 16, c[2] | c[3] << 16, c[4] | c[5] << 16, c[6] | c[7] << 16};
-            append(active && in != 0u && in != 255u, words, (uint32_t)tid | (uint32_t)(k - k0) << 11);
+            append(active && in != 0u && in != 255u, words, (uint32_t)tid | (uint32_t)(k - k0) << 14);
         } else {
             uint32_t diff = 0;
 #pragma unroll
             for (int m = 1; m < 8; ++m) diff |= c[m] ^ c[0];
             if (__ballot(active && diff != 0u) != 0ull) {   // some cube of the wave has differing corners
+                // per face: the 8 counts as nibbles, and whether the face's surface crosses the cube
+                uint32_t nib[NS];
+                unsigned surf = 0;
 #pragma unroll
                 for (int s = 0; s < NS; ++s) {
-                    uint32_t nib = 0;
+                    nib[s] = 0;
                     unsigned in = 0;
 #pragma unroll
                     for (int m = 0; m < 8; ++m) {
                         const uint32_t q = (c[m] >> (4 * s)) & 15u;
-                        nib |= q << (4 * m);
+                        nib[s] |= q << (4 * m);
                         in |= (q < thr ? 1u : 0u) << m;
                     }
-                    const uint32_t words[QW] = {nib};
-                    append(active && in != 0u && in != 255u, words,
-                           (uint32_t)tid | (uint32_t)s << 8 | (uint32_t)(k - k0) << 11);
+                    if (active && in != 0u && in != 255u) surf |= 1u << s;
+                }
+                // faces with equal counts have the same triangles: queue the cube once, with
+                // the mask of those faces (usually all six share the pore space here)
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    bool lead = ((surf >> s) & 1u) != 0u;
+                    unsigned fm = 0;
+#pragma unroll
+                    for (int t = 0; t < NS; ++t) {
+                        const bool same = ((surf >> t) & 1u) != 0u && nib[t] == nib[s];
+                        if (t < s && same) lead = false;
+                        if (t >= s && same) fm |= 1u << t;
+                    }
+                    const uint32_t words[QW] = {nib[s]};
+                    append(lead, words, (uint32_t)tid | fm << 8 | (uint32_t)(k - k0) << 14);
                 }
             }
         }
@@ -301,7 +317,7 @@ __global__ __launch_bounds__(kNT) void k_mc_area(const uint8_t* __restrict__ src
         for (int q = flushed ? tid : n; q < n; q += kNT) {
             const uint32_t h = QH[q];
             const int t = (int)(h & 255u);
-            g.Z0 = sz * (float)(k0 + (int)(h >> 11));
+            g.Z0 = sz * (float)(k0 + (int)(h >> 14));
             g.Z1 = g.Z0 + sz;
             g.X0 = sx * (float)(x0 + t % kTX);
             g.X1 = g.X0 + sx;
@@ -320,10 +336,10 @@ __global__ __launch_bounds__(kNT) void k_mc_area(const uint8_t* __restrict__ src
             if constexpr (MODE == 0) {
                 acc[0].add2(a, 0.0);
             } else {
-                const unsigned face = (h >> 8) & 7u;
+                const unsigned faces = (h >> 8) & 63u;
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
-                    if (face == (unsigned)s) acc[s].add2(a, 0.0);
+                    if ((faces >> s) & 1u) acc[s].add2(a, 0.0);
             }
         }
         cur ^= 1;
