@@ -310,7 +310,7 @@ int fast_debug_time_passes(fast_volume* vol, int dir, int precision, int solver,
  * whole iteration.  ms: 5 doubles. */
 int fast_debug_time_rb_passes(fast_volume* vol, int dir, int iters, double* ms);
 /* `fast alpha` kernels, each run `iters` times back to back on the context's stream (HIP
- * events): ms[0] = whole-volume area (area pass + final sum), ms[1] = the six per-face areas
+ * events) after one untimed run: ms[0] = whole-volume area (area pass + final sum), ms[1] = the six per-face areas
  * from the cached labels (face codes + area pass + final sums), ms[2] = the six per-face
  * areas from scratch (CCL labels, face table, face codes, area).  ms: 3 doubles. */
 int fast_debug_time_alpha(fast_volume* vol, int iters, double* ms);
