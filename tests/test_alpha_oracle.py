@@ -162,3 +162,29 @@ def test_cube_area_sums_to_total():
     m = np.where(rng.random((5, 6, 7)) < 0.5, 255, 0).astype(np.uint8)
     per = [O.mc_cube_area(m, x, y, z) for z in range(5) for y in range(6) for x in range(7)]
     assert math.fsum(per) == pytest.approx(O.mc_area(m), rel=1e-15)
+
+
+def test_segment_filter_and_ratio_host_logic():
+    """filterSegmentInfo keeps, with the inverted mask, the labels >= 1 that are not on the
+    given face row (row 6: not on every face); label 0 never; without inversion, the labels on
+    the row.  getCCLSegmentRatio is float(sum of voxels) / float(N)."""
+    import fast_amd as F
+    # pore regions: a slab open to x- only, a cavity sealed off, and a channel through x
+    m = np.full((6, 7, 9), 255, np.uint8)
+    m[1:3, 1:3, 0:4] = 0          # touches x- (and no other face)
+    m[4, 4, 4] = 0                # sealed
+    m[4, 1, :] = 0                # x- to x+
+    lab, nl = O.ccl(m)
+    table = O.boundary_table(lab, nl)
+    counts = np.bincount(lab.ravel().astype(np.int64), minlength=nl)
+    sinfo = [{"labelID": l, "voxelNum": int(counts[l])} for l in range(nl)]
+    for d in range(7):
+        kept = [s["labelID"] for s in F.filterSegmentInfo(sinfo, table, d)]
+        assert kept == [l for l in range(1, nl) if not table[d][l]]
+        assert [s["labelID"] for s in F.filterSegmentInfo(sinfo, table, d, invertMask=False)] == \
+            [l for l in range(nl) if table[d][l]]
+    sealed = lab[4, 4, 4]
+    none = F.filterSegmentInfo(sinfo, table, 6)
+    assert sealed in [s["labelID"] for s in none] and lab[4, 1, 0] in [s["labelID"] for s in none]
+    assert F.getCCLSegmentRatio(none, m.size) == float(np.float32(sum(s["voxelNum"] for s in none)) /
+                                                       np.float32(m.size))
