@@ -165,7 +165,8 @@ int fast_tau_solve_batch(fast_ctx* ctx, int count, fast_volume* const* vols,
 /* 6-connected labels of the reference's span set (background value `background`, 255 in
  * the reference callers), numbered 1..K by each component's minimum linear index, 0 for
  * background: VolumeCCL_Label (VolumeCCL.cu:381-620) via getVolumeCCL (VolumeSegmentation.cpp:33-64).
- * labels_out: N uint32 (nullable); num_labels = K + 1. */
+ * labels_out: N uint32 (nullable); num_labels = K + 1.  Volumes of 2^32 - 1 voxels or more are
+ * refused (labels are 32-bit), and so are the entry points below that label the volume. */
 int fast_ccl_labels(fast_volume* vol, uint8_t background, uint32_t* labels_out, uint32_t* num_labels);
 /* Reachable-pore mask for the axis of `dir`: 255 where the component touches both faces of
  * that axis, else 0 (VolumeCCL::getDirMask(begin,end), VolumeSegmentation.cpp:14-24, and
