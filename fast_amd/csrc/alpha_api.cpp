@@ -20,9 +20,9 @@ hipError_t mc_area(const uint8_t* src, int nx, int ny, int nz, float iso, int fa
                    hipStream_t s);
 hipError_t mesh_count(const uint8_t* src, int nx, int ny, int nz, float iso, uint8_t* padded, uint32_t* count,
                       hipStream_t s);
-hipError_t mesh_scan(const uint32_t* count, uint32_t* offset, int n, void* tmp, size_t* tmp_bytes, hipStream_t s);
+hipError_t mesh_scan(const uint32_t* count, uint64_t* offset, int n, void* tmp, size_t* tmp_bytes, hipStream_t s);
 hipError_t mesh_emit(const uint8_t* padded, int nx, int ny, int nz, float iso, const uint32_t* count,
-                     const uint32_t* offset, float* out, hipStream_t s);
+                     const uint64_t* offset, float* out, hipStream_t s);
 }  // namespace fta
 namespace ftc {
 hipError_t ccl_face_codes(const uint32_t* L, long long n, const uint8_t* table, uint32_t numLabels, uint8_t* out,
@@ -83,9 +83,9 @@ int face_codes(fast_volume* v, uint8_t** out) {
 int run_mesh(fast_volume* v, float iso, std::vector<float>* host, size_t capacity, float* out, size_t* nverts) {
     fast_ctx* ctx = v->ctx;
     const long long n = (long long)(v->nx + 2) * (v->ny + 2) * (v->nz + 2);
-    if (n > 0x7FFFFFFFLL || (unsigned long long)n * 15ull >= (1ull << 32)) {
-        set_error(ctx, "volume %dx%dx%d is too large for the mesh export (vertex offsets are 32-bit)", v->nx, v->ny,
-                  v->nz);
+    if (n > 0x7FFFFFFFLL) {   // the scan counts cubes in an int
+        set_error(ctx, "volume %dx%dx%d is too large for the mesh export (2^31 padded cubes or more)", v->nx,
+                  v->ny, v->nz);
         return FAST_E_BADARG;
     }
     size_t tmp_bytes = 0;
@@ -93,20 +93,26 @@ int run_mesh(fast_volume* v, float iso, std::vector<float>* host, size_t capacit
     if (!hip_ok(ctx, e, "mesh scan size")) return FAST_E_HIP;
     uint8_t* padded = (uint8_t*)dalloc(ctx, (size_t)n);
     uint32_t* count = (uint32_t*)dalloc(ctx, (size_t)n * 4);
-    uint32_t* offset = (uint32_t*)dalloc(ctx, (size_t)n * 4);
+    uint64_t* offset = (uint64_t*)dalloc(ctx, (size_t)n * 8);
     void* tmp = dalloc(ctx, tmp_bytes);
     float* d_out = nullptr;
     int rc = (padded && count && offset && tmp) ? FAST_OK : FAST_E_OOM;
-    uint32_t last[2] = {0, 0};
+    uint64_t last_off = 0;
+    uint32_t last_count = 0;
     if (!rc) {
         e = fta::mesh_count(v->d_mask, v->nx, v->ny, v->nz, iso, padded, count, ctx->stream);
         if (e == hipSuccess) e = fta::mesh_scan(count, offset, (int)n, tmp, &tmp_bytes, ctx->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(&last[0], offset + n - 1, 4, hipMemcpyDeviceToHost, ctx->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(&last[1], count + n - 1, 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(&last_off, offset + n - 1, 8, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(&last_count, count + n - 1, 4, hipMemcpyDeviceToHost, ctx->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         if (!hip_ok(ctx, e, "mesh count")) rc = FAST_E_HIP;
     }
-    const size_t total = (size_t)last[0] + last[1];
+    const size_t total = (size_t)(last_off + last_count);
+    // the reference numbers vertices with unsigned ints (VolumeSurface.cu:260-357)
+    if (!rc && total >= ((size_t)1 << 32)) {
+        set_error(ctx, "mesh of %zu vertices exceeds the 2^32 - 1 vertices of the reference's OBJ export", total);
+        rc = FAST_E_BADARG;
+    }
     if (!rc) *nverts = total;
     const bool want = !rc && total > 0 && (host || (out && capacity >= total));
     if (want) {

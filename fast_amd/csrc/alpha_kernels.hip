@@ -486,7 +486,7 @@ __global__ __launch_bounds__(256) void k_mesh_count(const uint8_t* __restrict__ 
 // out: 6 floats per vertex (position, then the triangle's normal), at 6 * offset[cube].
 __global__ __launch_bounds__(256) void k_mesh_emit(const uint8_t* __restrict__ v, int r0, int r1, int r2, float iso,
                                                    const uint32_t* __restrict__ count,
-                                                   const uint32_t* __restrict__ offset, float* __restrict__ out) {
+                                                   const uint64_t* __restrict__ offset, float* __restrict__ out) {
     const long long n = (long long)r0 * r1 * r2;
     const float tvs = 1.0f / (float)max(r0, max(r1, r2));   // params.dx = dy = dz = 1 / maxDim
     const float hx = 0.5f * (tvs * (float)r0), hy = 0.5f * (tvs * (float)r1), hz = 0.5f * (tvs * (float)r2);
@@ -553,13 +553,21 @@ hipError_t mesh_count(const uint8_t* src, int nx, int ny, int nz, float iso, uin
     return hipGetLastError();
 }
 
-// Exclusive prefix sum of n per-cube counts (hipCUB).  Call with tmp = nullptr for *tmp_bytes.
-hipError_t mesh_scan(const uint32_t* count, uint32_t* offset, int n, void* tmp, size_t* tmp_bytes, hipStream_t s) {
-    return hipcub::DeviceScan::ExclusiveSum(tmp, *tmp_bytes, count, offset, n, s);
+namespace {
+struct Widen {
+    __host__ __device__ uint64_t operator()(uint32_t v) const { return v; }
+};
+}  // namespace
+
+// Exclusive prefix sum of n per-cube counts into 64-bit offsets (hipCUB, summing in 64 bits).
+// Call with tmp = nullptr for *tmp_bytes.
+hipError_t mesh_scan(const uint32_t* count, uint64_t* offset, int n, void* tmp, size_t* tmp_bytes, hipStream_t s) {
+    hipcub::TransformInputIterator<uint64_t, Widen, const uint32_t*> wide(count, Widen());
+    return hipcub::DeviceScan::ExclusiveSum(tmp, *tmp_bytes, wide, offset, n, s);
 }
 
 hipError_t mesh_emit(const uint8_t* padded, int nx, int ny, int nz, float iso, const uint32_t* count,
-                     const uint32_t* offset, float* out, hipStream_t s) {
+                     const uint64_t* offset, float* out, hipStream_t s) {
     const long long n = (long long)(nx + 2) * (ny + 2) * (nz + 2);
     hipLaunchKernelGGL(k_mesh_emit, dim3(mesh_grid(n)), dim3(256), 0, s, padded, nx + 2, ny + 2, nz + 2, iso, count,
                        offset, out);

@@ -27,11 +27,9 @@ int ensure_labels(fast_volume* v, uint8_t background, uint32_t* num_labels) {
         *num_labels = v->num_labels;
         return FAST_OK;
     }
+    // labels and span starts are 32-bit voxel indices (all-ones marks "no span"); volumes hold
+    // fewer than 2^31 voxels (fast_volume_create refuses larger ones), so every index fits
     const long long n = (long long)v->nx * v->ny * v->nz;
-    if (n >= 0xFFFFFFFFLL) {   // labels and span starts are 32-bit voxel indices; all-ones marks "no span"
-        set_error(ctx, "volume of %d x %d x %d voxels is too large for the 32-bit labels", v->nx, v->ny, v->nz);
-        return FAST_E_BADARG;
-    }
     if (!v->d_labels) v->d_labels = (uint32_t*)dalloc(ctx, (size_t)n * 4);
     uint32_t* P = (uint32_t*)dalloc(ctx, (size_t)n * 4);
     const long long nb = (n + 4095) / 4096;

@@ -55,20 +55,20 @@ __device__ __forceinline__ void merge(uint32_t* P, uint32_t a, uint32_t b) {
     }
 }
 
-// Span starts, one wave per x-row in chunks of 64 voxels.  A voxel is in a span when the
-// number of value changes up to it (from `background` before the row) is odd, and its span
-// starts at the last change at or before it; both come from a ballot of the change flags,
-// with the parity and the last change carried from chunk to chunk.
+// Span starts, one wave per x-row in chunks of 64 voxels; the waves stride over the rows, so
+// the grid stays within the launch limit however many rows there are.  A voxel is in a span
+// when the number of value changes up to it (from `background` before the row) is odd, and
+// its span starts at the last change at or before it; both come from a ballot of the change
+// flags, with the parity and the last change carried from chunk to chunk.
 __global__ __launch_bounds__(256) void k_spans(const uint8_t* __restrict__ mask, uint32_t* __restrict__ P, int nx,
                                                int rows, uint8_t background) {
     const int lane = threadIdx.x & 63;
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (row >= rows) return;   // the whole wave
-    const long long base = (long long)row * nx;
+    const unsigned long long upto = (2ull << lane) - 1ull;   // lanes 0..lane
+    for (long long row = (long long)blockIdx.x * 4 + (threadIdx.x >> 6); row < rows; row += (long long)gridDim.x * 4) {
+    const long long base = row * nx;
     int prev_last = background;   // value before this chunk
     unsigned parity = 0;          // number of changes before this chunk, mod 2
     uint32_t last_change = kNone; // index of the last change before this chunk
-    const unsigned long long upto = (2ull << lane) - 1ull;   // lanes 0..lane
     for (int x0 = 0; x0 < nx; x0 += 64) {
         const int x = x0 + lane;
         const bool in = x < nx;
@@ -83,6 +83,7 @@ __global__ __launch_bounds__(256) void k_spans(const uint8_t* __restrict__ mask,
         parity = (parity + (unsigned)__popcll(changes)) & 1u;
         if (changes) last_change = (uint32_t)(base + x0 + (63 - __clzll((long long)changes)));
         prev_last = __shfl(v, 63, 64);
+    }
     }
 }
 
@@ -314,8 +315,9 @@ inline unsigned grid_for(long long n) {
 hipError_t ccl_label(const uint8_t* mask, int nx, int ny, int nz, uint8_t background, uint32_t* P, uint32_t* L,
                      uint32_t* blocksum, uint32_t* num_roots, hipStream_t s) {
     const long long n = (long long)nx * ny * nz;
-    const int rows = ny * nz;
-    hipLaunchKernelGGL(k_spans, dim3((rows + 3) / 4), dim3(256), 0, s, mask, P, nx, rows, background);
+    const int rows = ny * nz;   // < 2^31: volumes hold fewer than 2^31 voxels
+    const int span_blocks = std::min((rows + 3) / 4, 65536);   // 4 rows per block and sweep
+    hipLaunchKernelGGL(k_spans, dim3(span_blocks), dim3(256), 0, s, mask, P, nx, rows, background);
     hipLaunchKernelGGL(k_union, dim3(grid_for(n)), dim3(256), 0, s, mask, P, nx, ny, nz);
     hipLaunchKernelGGL(k_flatten, dim3(grid_for(n)), dim3(256), 0, s, P, n);
     const int nb = (int)((n + kScanChunk - 1) / kScanChunk);

@@ -458,6 +458,7 @@ int retire(fast_server* s, const std::vector<int>& fin) {
     size_t pos = 0;
     const double t = now_s();
     std::vector<Req*> out;
+    out.reserve(fin.size());   // no allocation between taking a request out of its slot and listing it
     for (int i : fin) {
         Req* r = s->slots[i];
         const VolState<double>& q = s->h_st[i];
@@ -488,9 +489,9 @@ int retire(fast_server* s, const std::vector<int>& fin) {
         dfree(ctx, r->d_mask);
         dfree(ctx, r->d_code);
         r->d_mask = r->d_code = nullptr;
+        out.push_back(r);
         s->arena.release(r->off, r->elems);
         s->slots[i] = nullptr;
-        out.push_back(r);
     }
     {
         std::lock_guard<std::mutex> lk(s->mu);
@@ -500,7 +501,13 @@ int retire(fast_server* s, const std::vector<int>& fin) {
     return FAST_OK;
 }
 
+// Fails every admitted and queued request with `rc`.  Admitted requests give back their device
+// mask and code and their arena range first (after the stream has drained, so no kernel still
+// reads them); the worker stops after this.
 void fail_all(fast_server* s, int rc) {
+    fast_ctx* ctx = s->ctx;
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipGetLastError();
     std::lock_guard<std::mutex> lk(s->mu);
     s->fatal = rc;
     auto finish = [&](Req* r) {
@@ -511,6 +518,13 @@ void fail_all(fast_server* s, int rc) {
     };
     for (Req*& r : s->slots)
         if (r) {
+            dfree(ctx, r->d_mask);
+            dfree(ctx, r->d_code);
+            r->d_mask = r->d_code = nullptr;
+            try {
+                s->arena.release(r->off, r->elems);
+            } catch (const std::bad_alloc&) {   // the arena is not used again once the server has failed
+            }
             finish(r);
             r = nullptr;
         }

@@ -165,8 +165,8 @@ int fast_tau_solve_batch(fast_ctx* ctx, int count, fast_volume* const* vols,
 /* 6-connected labels of the reference's span set (background value `background`, 255 in
  * the reference callers), numbered 1..K by each component's minimum linear index, 0 for
  * background: VolumeCCL_Label (VolumeCCL.cu:381-620) via getVolumeCCL (VolumeSegmentation.cpp:33-64).
- * labels_out: N uint32 (nullable); num_labels = K + 1.  Volumes of 2^32 - 1 voxels or more are
- * refused (labels are 32-bit), and so are the entry points below that label the volume. */
+ * labels_out: N uint32 (nullable); num_labels = K + 1.  Labels are 32-bit voxel indices, which
+ * every volume fits (fast_volume_create takes fewer than 2^31 voxels). */
 int fast_ccl_labels(fast_volume* vol, uint8_t background, uint32_t* labels_out, uint32_t* num_labels);
 /* Reachable-pore mask for the axis of `dir`: 255 where the component touches both faces of
  * that axis, else 0 (VolumeCCL::getDirMask(begin,end), VolumeSegmentation.cpp:14-24, and
@@ -211,8 +211,9 @@ int fast_alpha_area_tensor(fast_volume* vol, float isovalue, double area[6]);
  * positions in a box centred at the origin with spacing 1/maxDim of the padded dims, three
  * vertices per triangle in cube order then table order, each carrying its triangle's unit
  * normal.  *nverts = number of vertices; when verts is non-null and capacity (in vertices) is
- * at least that, verts receives 6 floats per vertex (x, y, z, nx, ny, nz).  Volumes whose
- * padded cube count times 15 reaches 2^32 are refused. */
+ * at least that, verts receives 6 floats per vertex (x, y, z, nx, ny, nz).  Meshes of 2^32
+ * vertices or more (the reference's unsigned vertex count) and padded volumes of 2^31 cubes or
+ * more are refused. */
 int fast_alpha_mesh(fast_volume* vol, float isovalue, float* verts, size_t capacity, size_t* nverts);
 /* The same mesh written as the reference's OBJ (CUDA_VBO::saveObj, DataPtr.cpp:684-740):
  * vertices merged by equal position in order of first appearance, normals summed and

@@ -42,6 +42,22 @@ def test_ccl_and_reachable_match_oracle(ctx, shape, frac, seed):
         assert np.array_equal(out, oout), d
 
 
+def test_ccl_thin_x_volume_with_more_than_2pow26_rows(ctx):
+    """1 x 8192 x 8193 voxels: 67,117,056 x-rows, above 2^26, so the span pass (one wave per
+    row) needs more waves than one launch may hold and strides over the rows."""
+    rng = np.random.default_rng(11)
+    m = ((rng.random((8193, 8192, 1)) < 0.35) * 255).astype(np.uint8)   # percolates in z
+    print("thin-x CCL: 1 x 8192 x 8193", flush=True)
+    vol = ctx.volume(m)
+    lab, nl = vol.ccl_labels(255)
+    olab, onl = O.ccl(m)
+    assert nl == onl and np.array_equal(lab, olab)
+    out, _ = vol.reachable_mask(4, 255)
+    oout, _, _ = O.reachable(m, 4)
+    assert np.array_equal(out, oout)
+    vol.close()
+
+
 @pytest.mark.parametrize("shape,seed", [((40, 36, 70), 4), ((9, 130, 66), 5)])
 def test_ccl_multivalued_bytes_match_oracle(ctx, shape, seed):
     """Bytes 0, 100 and 255 in runs: every change of value toggles a span, so equal
