@@ -228,6 +228,12 @@ void assemble(const uint8_t* mask, const Dims& d, int dir, int threads, CSR<T>& 
     }
 }
 
+// Elementwise vector updates (Eigen runs them as single-thread SIMD loops) are split over the
+// OpenMP threads: each element gets the same unfused operations in the same order, so the
+// results are bit-identical to a serial loop, and 512^3 solves take minutes less per hundred
+// iterations.
+#define ELEMWISE _Pragma("omp parallel for schedule(static)")
+
 // Eigen sparse (row-major) * dense: tmp = 0; tmp += a * x over the row's entries in order.
 template <class T>
 void spmv(const CSR<T>& A, const T* x, T* y) {
@@ -343,26 +349,26 @@ SolveOut bicgstab(const CSR<T>& A, std::vector<T>& x, T tol, long maxIters) {
         rho = dot(r0, r);
         if (std::abs(rho) < eps2 * r0_sqnorm) {
             spmv(A, x.data(), tmpv.data());
-            for (size_t k = 0; k < n; ++k) r[k] = A.b[k] - tmpv[k];
+            ELEMWISE for (int64_t k = 0; k < (int64_t)n; ++k) r[k] = A.b[k] - tmpv[k];
             r0 = r;
             rho = r0_sqnorm = sqnorm(r);
             if (restarts++ == 0) i = 0;
         }
         const T beta = (rho / rho_old) * (alpha / w);
-        for (size_t k = 0; k < n; ++k) p[k] = r[k] + beta * (p[k] - w * v[k]);
-        for (size_t k = 0; k < n; ++k) y[k] = inv[k] * p[k];
+        ELEMWISE for (int64_t k = 0; k < (int64_t)n; ++k) p[k] = r[k] + beta * (p[k] - w * v[k]);
+        ELEMWISE for (int64_t k = 0; k < (int64_t)n; ++k) y[k] = inv[k] * p[k];
         spmv(A, y.data(), v.data());
         alpha = rho / dot(r0, v);
-        for (size_t k = 0; k < n; ++k) s[k] = r[k] - alpha * v[k];
-        for (size_t k = 0; k < n; ++k) z[k] = inv[k] * s[k];
+        ELEMWISE for (int64_t k = 0; k < (int64_t)n; ++k) s[k] = r[k] - alpha * v[k];
+        ELEMWISE for (int64_t k = 0; k < (int64_t)n; ++k) z[k] = inv[k] * s[k];
         spmv(A, z.data(), t.data());
         const T tmp = sqnorm(t);
         if (tmp > T(0))
             w = dot(t, s) / tmp;
         else
             w = T(0);
-        for (size_t k = 0; k < n; ++k) x[k] += alpha * y[k] + w * z[k];
-        for (size_t k = 0; k < n; ++k) r[k] = s[k] - w * t[k];
+        ELEMWISE for (int64_t k = 0; k < (int64_t)n; ++k) x[k] += alpha * y[k] + w * z[k];
+        ELEMWISE for (int64_t k = 0; k < (int64_t)n; ++k) r[k] = s[k] - w * t[k];
         ++i;
     }
     const T err = std::sqrt(sqnorm(r) / rhs_sqnorm);
@@ -406,15 +412,15 @@ SolveOut cg(const CSR<T>& A, std::vector<T>& x, T tol, long maxIters) {
     while (i < maxIters) {
         spmv(A, p.data(), tmp.data());
         const T alpha = absNew / dot(p, tmp);
-        for (size_t k = 0; k < n; ++k) x[k] += alpha * p[k];
-        for (size_t k = 0; k < n; ++k) residual[k] -= alpha * tmp[k];
+        ELEMWISE for (int64_t k = 0; k < (int64_t)n; ++k) x[k] += alpha * p[k];
+        ELEMWISE for (int64_t k = 0; k < (int64_t)n; ++k) residual[k] -= alpha * tmp[k];
         residualNorm2 = sqnorm(residual);
         if (residualNorm2 < threshold) break;
-        for (size_t k = 0; k < n; ++k) z[k] = inv[k] * residual[k];
+        ELEMWISE for (int64_t k = 0; k < (int64_t)n; ++k) z[k] = inv[k] * residual[k];
         const T absOld = absNew;
         absNew = dot(residual, z);
         const T beta = absNew / absOld;
-        for (size_t k = 0; k < n; ++k) p[k] = z[k] + beta * p[k];
+        ELEMWISE for (int64_t k = 0; k < (int64_t)n; ++k) p[k] = z[k] + beta * p[k];
         ++i;
     }
     const T err = std::sqrt(residualNorm2 / rhsNorm2);

@@ -7,6 +7,10 @@ SHA-256 of the generated mask so a test can prove it solved the same volume.
 
   python tests/golden/make_oracle_fixtures.py config2             # 256^3 ellipsoids, CG + BiCGSTAB, X/Y/Z
   python tests/golden/make_oracle_fixtures.py config2_converged   # same volume, CG to 1e-11
+  python tests/golden/make_oracle_fixtures.py config4             # 96 uneven volumes, CG to 1e-11,
+                                                                   # plus CG/BiCGSTAB 1e-6 on four of them
+  python tests/golden/make_oracle_fixtures.py config3             # 512^3 needles, CG 1e-6 and 1e-11 (x),
+                                                                   # then CG 1e-11 (y, z)
 """
 from __future__ import annotations
 
@@ -31,23 +35,68 @@ def config2_mask():
     return F.generate_ellipsoids(256, 12.0, 0.35, 0.65, 0)
 
 
-def run(name: str, mask, cases, tol=1e-6, threads=0):
+DIRS = ['X_POS', 'X_NEG', 'Y_POS', 'Y_NEG', 'Z_POS', 'Z_NEG']
+
+
+def _solve(mask, solver, d, tol, max_iter, threads):
+    digest = hashlib.sha256(mask.tobytes()).hexdigest()
+    t = time.time()
+    r, _ = O.tau(mask, d, solver=solver, precision=0, tol=tol, max_iter=max_iter, threads=threads)
+    return {
+        "sha256": digest, "dims": list(mask.shape[::-1]), "solver": solver, "dir": d, "tol": tol,
+        "max_iter": max_iter, "tau": r.tau, "porosity": r.porosity, "iterations": r.iterations, "error": r.error,
+        "dc": r.dc, "status": r.status, "restarts": r.restarts, "seconds": time.time() - t,
+    }
+
+
+def run(name: str, mask, cases, tol=1e-6, threads=0, max_iter=10000):
     out_path = HERE / f"{name}_oracle.json"
     recs = json.loads(out_path.read_text()) if out_path.exists() else {}
     digest = hashlib.sha256(mask.tobytes()).hexdigest()
     for solver, d in cases:
-        key = f"{['BICGSTAB', 'CG'][solver]}_{['X_POS', 'X_NEG', 'Y_POS', 'Y_NEG', 'Z_POS', 'Z_NEG'][d]}"
-        if key in recs and recs[key]["sha256"] == digest:
+        key = f"{['BICGSTAB', 'CG'][solver]}_{DIRS[d]}"
+        if key in recs and recs[key]["sha256"] == digest and recs[key]["tol"] == tol:
             continue
-        t = time.time()
-        r, _ = O.tau(mask, d, solver=solver, precision=0, tol=tol, max_iter=10000, threads=threads)
-        recs[key] = {
-            "sha256": digest, "dims": list(mask.shape[::-1]), "solver": solver, "dir": d, "tol": tol,
-            "tau": r.tau, "porosity": r.porosity, "iterations": r.iterations, "error": r.error, "dc": r.dc,
-            "status": r.status, "restarts": r.restarts, "seconds": time.time() - t,
-        }
+        recs[key] = _solve(mask, solver, d, tol, max_iter, threads)
         out_path.write_text(json.dumps(recs, indent=1, sort_keys=True))
         print(key, recs[key], flush=True)
+
+
+def config4(threads=0):
+    """The 96-volume batch (fast_amd.workloads.uneven_batch(96, 1000)), direction x, y, z by
+    index mod 3 as in the GPU batch test.  'converged': CG to 1e-11 for every volume;
+    'fp64': CG and BiCGSTAB at 1e-6 on volumes 0, 17, 42 and 95 (the GPU FP64 path must give
+    the same iterations and tau bit for bit)."""
+    from fast_amd import workloads as W
+    out_path = HERE / "config4_oracle.json"
+    recs = json.loads(out_path.read_text()) if out_path.exists() else {"converged": {}, "fp64": {}}
+    items = W.uneven_batch(96, 1000)
+    for i, b in enumerate(items):
+        d = (0, 2, 4)[i % 3]
+        key = str(i)
+        if key not in recs["converged"]:
+            recs["converged"][key] = _solve(b.mask, 1, d, 1e-11, 60000, threads)
+            recs["converged"][key].update(r=b.r, solid=b.solid, seed=b.seed)
+            out_path.write_text(json.dumps(recs, indent=1, sort_keys=True))
+            print("converged", i, recs["converged"][key], flush=True)
+    for i in (0, 17, 42, 95):
+        d = (0, 2, 4)[i % 3]
+        for solver in (0, 1):
+            key = f"{i}_{['BICGSTAB', 'CG'][solver]}"
+            if key not in recs["fp64"]:
+                recs["fp64"][key] = _solve(items[i].mask, solver, d, 1e-6, 20000, threads)
+                out_path.write_text(json.dumps(recs, indent=1, sort_keys=True))
+                print("fp64", key, recs["fp64"][key], flush=True)
+
+
+def config3(threads=0):
+    """512^3 needles (a = 12, r = 0.1, solid 0.55, seed 0): CG to 1e-11 (the converged tau)
+    and CG at 1e-6 (bit identity with the GPU FP64 path) for x, then CG to 1e-11 for y and z."""
+    from fast_amd import workloads as W
+    mask = W.needles_512(0)
+    run("config3_converged", mask, [(1, 0)], tol=1e-11, threads=threads, max_iter=40000)
+    run("config3", mask, [(1, 0)], tol=1e-6, threads=threads, max_iter=40000)
+    run("config3_converged", mask, [(1, 2), (1, 4)], tol=1e-11, threads=threads, max_iter=40000)
 
 
 if __name__ == "__main__":
@@ -58,3 +107,9 @@ if __name__ == "__main__":
         # the converged tau (CG to 1e-11): the yardstick for tau accuracy at 256^3, where
         # BiCGSTAB stopped at 1e-6 is still up to ~1e-4 away from it
         run("config2_converged", config2_mask(), [(1, 0), (1, 2), (1, 4)], tol=1e-11)
+    elif which == "config4":
+        config4()
+    elif which == "config3":
+        config3()
+    else:
+        sys.exit(f"unknown fixture set {which!r}")
