@@ -18,6 +18,7 @@ bool mc_fits(int nx, int ny, int nz);
 size_t mc_blocks(int nx, int ny, int nz);
 hipError_t mc_area(const uint8_t* src, int nx, int ny, int nz, float iso, int faces, double* part, double* out,
                    hipStream_t s);
+size_t mc_scratch_doubles();
 hipError_t mesh_count(const uint8_t* src, int nx, int ny, int nz, float iso, uint8_t* padded, uint32_t* count,
                       hipStream_t s);
 hipError_t mesh_scan(const uint32_t* count, uint64_t* offset, int n, void* tmp, size_t* tmp_bytes, hipStream_t s);
@@ -219,6 +220,27 @@ int fast_alpha_area(fast_volume* v, float isovalue, double* area) {
     (void)hipSetDevice(ctx->device);
     if (!iso_ok(ctx, isovalue)) return FAST_E_BADARG;
     return run_area(v, v->d_mask, isovalue, 0, area);
+}
+
+int fast_alpha_area_noalloc(fast_volume* v, float isovalue, double* area) {
+    if (!v || !area) return FAST_E_BADARG;
+    fast_ctx* ctx = v->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    if (!iso_ok(ctx, isovalue)) return FAST_E_BADARG;
+    const size_t cap = fta::mc_scratch_doubles();
+    if (!fta::mc_fits(v->nx, v->ny, v->nz) || fta::mc_blocks(v->nx, v->ny, v->nz) * 2 + 1 > cap) {
+        set_error(ctx, "volume %dx%dx%d needs more than the %zu area partials of the static scratch "
+                  "(at most %zu blocks of 64x4x16 voxels)", v->nx, v->ny, v->nz, (cap - 1) / 2, (cap - 1) / 2);
+        return FAST_E_BADARG;
+    }
+    // block partials, then the result, in this device's static scratch (set up with the context)
+    double* part = ctx->alpha_scratch;
+    double* d_out = part + fta::mc_blocks(v->nx, v->ny, v->nz) * 2;
+    hipError_t e = fta::mc_area(v->d_mask, v->nx, v->ny, v->nz, isovalue, 0, part, d_out, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(area, d_out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return hip_ok(ctx, e, "marching-cubes area (static scratch)") ? FAST_OK : FAST_E_HIP;
 }
 
 int fast_alpha_area_tensor(fast_volume* v, float isovalue, double area[6]) {

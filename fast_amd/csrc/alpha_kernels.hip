@@ -32,6 +32,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <cstdint>
+#include <mutex>
 
 #include "mc_table.h"
 
@@ -387,6 +388,35 @@ __global__ __launch_bounds__(256) void k_mc_final(const double* __restrict__ par
         for (int q = 0; q < 4; ++q) t.add2(red[q][0], red[q][1]);
         out[s] = t.hi + t.lo;
     }
+}
+
+// Scratch of the allocation-free whole-volume area (fast_alpha_area_noalloc): the block
+// partials (2 doubles per block of k_mc_area<0>) followed by the result (1 double).  Every
+// block writes its own partial and k_mc_final writes the result, so nothing in it needs to
+// start at zero.  One copy per device, held until the process ends; calls that overlap on one
+// device share it.
+constexpr size_t kScratchDoubles = (size_t)1 << 17;   // 1 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+size_t mc_scratch_doubles() { return kScratchDoubles; }
+
+// Device address of `scratch` on `device` (the current device): looked up with
+// hipGetSymbolAddress on the first call for that device and kept.  Host code uses only this
+// address; the array's name is not a device address in host code.
+hipError_t mc_scratch_address(int device, double** out) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static double* addr[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        addr[device] = (double*)p;
+    }
+    *out = addr[device];
+    return hipSuccess;
 }
 
 // Grid of the area kernel for an nx x ny x nz volume.

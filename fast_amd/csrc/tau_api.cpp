@@ -28,6 +28,10 @@
 
 using namespace ft;
 
+namespace fta {
+hipError_t mc_scratch_address(int device, double** out);   // alpha_kernels.hip
+}  // namespace fta
+
 // ------------------------------------------------------------------------------------------
 // context and device memory
 // ------------------------------------------------------------------------------------------
@@ -265,6 +269,11 @@ int fast_ctx_create(int device, size_t hbm_budget, fast_ctx** out) {
     if (!ctx->zero_count) {
         fast_ctx_destroy(ctx);
         return FAST_E_OOM;
+    }
+    if (fta::mc_scratch_address(device, &ctx->alpha_scratch) != hipSuccess) {
+        (void)hipGetLastError();
+        fast_ctx_destroy(ctx);
+        return FAST_E_HIP;
     }
     load_tuning(ctx);
     if (const char* e = std::getenv("FAST_TAU_MIXED_RESTART")) ctx->mixed_restart_cos = std::atof(e);

@@ -44,6 +44,7 @@ struct fast_ctx {
     hipEvent_t ev[2] = {nullptr, nullptr};
     hipEvent_t t0 = nullptr, t1 = nullptr;
     unsigned long long* zero_count = nullptr;
+    double* alpha_scratch = nullptr;           // this device's copy of the area kernels' static scratch
     std::map<std::string, fth::Shape> tuned;   // autotuned tile shapes by size class
     fth::Shape forced{0, 0, 0};                // tile shape forced by fast_ctx_set_tile_shape
     int chunk = 16;                            // Krylov iterations per polling round

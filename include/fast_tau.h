@@ -200,6 +200,15 @@ int fast_ccl_segment_info(fast_volume* vol, uint8_t background, fast_segment_inf
  * getVolumeArea (VolumeSurface.cpp:9-36) -> VolumeSurface_MarchingCubesArea
  * (VolumeSurface.cu:203-258, 507-535) and Volume::sum. */
 int fast_alpha_area(fast_volume* vol, float isovalue, double* area);
+/* fast_alpha_area without any device allocation.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc).  The kernels' block partials and result live in a static 1 MiB
+ * device array (one per device, set up when the context is created, held until the process
+ * ends), so volumes needing more than 65,535 blocks of 64x4x16 voxels return FAST_E_BADARG
+ * (512^3 takes 32,768 blocks and fits; 512x512x1024 takes 65,536 and does not).  Same kernels and the same result as fast_alpha_area, bit for bit.
+ * Calls that overlap on one device race on the shared array (one context serialises its own
+ * calls). */
+int fast_alpha_area_noalloc(fast_volume* vol, float isovalue, double* area);
 /* The six per-face areas of getReactiveAreaDensityTensor<double>(getVolumeCCL(vol, 255))
  * (VolumeMeasures.cpp:303-333): for face f (Dir order x+, x-, y+, y-, z+, z-), the area
  * above on the volume that is 255 on the pore voxels whose component touches face f and 0
