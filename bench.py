@@ -92,30 +92,46 @@ def cpu_baseline(mask: np.ndarray, gpu_iters: float, threads: int, sample_iters:
     }
 
 
-def serving(ctx, T, W, n_req: int, loads=(0.5, 0.8, 1.1)) -> dict:
+SERVE_TOL = 1e-9   # FP64 true relative residual of served requests (tau within 1e-5 of converged)
+
+
+def serving(ctx, T, W, n_req: int, loads=(0.5, 0.8, 1.1), per_edge: int = 16, rate: float = 0.0,
+            check_tau: bool = True) -> dict:
     """Continuous-batching server under Poisson arrivals.  Requests: cubic edge from
     {64,128,192,256,384} (weights .3/.3/.2/.15/.05), direction uniform in x/y/z, ellipsoid
-    volumes (r log-uniform on [0.1, 10], solid 0.5..0.75) from a pool of 4 per edge; mixed CG
-    to tol 1e-6.  The saturated rate is measured with a burst; then `n_req` requests arrive at
-    each offered load (fraction of that rate).  Latency = completion - submission."""
+    volumes (r log-uniform on [0.1, 10], solid 0.5..0.75) from a pool of `per_edge` per edge;
+    mixed CG (red-black) to an FP64 true residual of SERVE_TOL.  The saturated rate is measured
+    with a burst (or given as `rate`); then `n_req` requests arrive at each offered load
+    (fraction of that rate).  Latency = completion - submission.  Afterwards every (volume,
+    direction) pair served is solved in FP64 CG to 1e-11 and each request's tau error against
+    it is reported."""
     import threading
-    pool = W.request_pool(4)
-    mk = lambda d: T.make_params(d, CG, "mixed", TOL, 20000)
-    out = {"requests_per_load": n_req, "volume_pool": "4 volumes per edge", "tolerance": TOL, "solver": "mixed CG"}
+    pool = W.request_pool(per_edge, threads=16)
+    mk = lambda d: T.make_params(d, CG, "mixed", SERVE_TOL, 40000)
+    out = {"requests_per_load": n_req, "volume_pool": f"{per_edge} volumes per edge, {len(pool)} in all",
+           "tolerance": SERVE_TOL, "solver": "mixed CG (red-black Schur complement)"}
+    served = []   # (load index, edge, volume, dir, latency ms, tau, status)
     with T.Server(ctx, max_active=64, iters_per_round=16) as srv:
-        burst = W.request_stream(max(n_req // 2, 24), 1e9, seed=11)
-        t0 = time.perf_counter()
-        for q in burst:
-            srv.submit(pool[(q.edge, q.volume)], mk(q.dir))
-        for _ in burst:
-            srv.wait(-1)
-        sat = len(burst) / (time.perf_counter() - t0)
-        out["saturated_solves_per_s"] = sat
-        log(f"[bench] serving: saturated {sat:.1f} solves/s over a burst of {len(burst)}")
+        if rate > 0:
+            sat = rate
+            out["saturated_solves_per_s"] = sat
+            out["saturated_source"] = "given"
+        else:
+            burst = W.request_stream(max(n_req // 2, 24), 1e9, per_edge=per_edge, seed=11)
+            t0 = time.perf_counter()
+            for q in burst:
+                srv.submit(pool[(q.edge, q.volume)], mk(q.dir))
+            for _ in burst:
+                srv.wait(-1)
+            sat = len(burst) / (time.perf_counter() - t0)
+            out["saturated_solves_per_s"] = sat
+            out["saturated_source"] = f"burst of {len(burst)} requests"
+            log(f"[bench] serving: saturated {sat:.1f} solves/s over a burst of {len(burst)}")
         runs = []
         for li, f in enumerate(loads):
-            reqs = W.request_stream(n_req, f * sat, seed=100 + li)
-            lat = []
+            reqs = W.request_stream(n_req, f * sat, per_edge=per_edge, seed=100 + int(round(10 * f)))
+            tickets = {}
+            lock = threading.Lock()
             start = time.perf_counter()
 
             def feed():
@@ -123,21 +139,55 @@ def serving(ctx, T, W, n_req: int, loads=(0.5, 0.8, 1.1)) -> dict:
                     dt = start + q.arrival - time.perf_counter()
                     if dt > 0:
                         time.sleep(dt)
-                    srv.submit(pool[(q.edge, q.volume)], mk(q.dir))
+                    t = srv.submit(pool[(q.edge, q.volume)], mk(q.dir))
+                    with lock:
+                        tickets[t] = q
 
             th = threading.Thread(target=feed)
             th.start()
-            for _ in reqs:
-                _, res, ms = srv.wait(-1)
-                lat.append(ms)
+            done = []
+            for k in range(len(reqs)):
+                done.append(srv.wait(-1))
+                if (k + 1) % 250 == 0:
+                    log(f"[bench] serving load {f:.2f}: {k + 1}/{len(reqs)} done")
             th.join()
             wall = time.perf_counter() - start
+            lat = []
+            for t, res, ms in done:
+                q = tickets[t]
+                served.append((li, q.edge, q.volume, q.dir, ms, res.tau, res.status))
+                lat.append(ms)
+            byedge = {}
+            for e in W.REQUEST_EDGES:
+                le = [x[4] for x in served if x[0] == li and x[1] == e]
+                if le:
+                    byedge[str(e)] = {"n": len(le), "p50_ms": float(np.percentile(le, 50)),
+                                      "p99_ms": float(np.percentile(le, 99)), "max_ms": float(np.max(le))}
             runs.append({"offered_fraction": f, "offered_solves_per_s": f * sat, "achieved_solves_per_s": n_req / wall,
                          "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-                         "mean_ms": float(np.mean(lat))})
-            log(f"[bench] serving load {f:.1f}: p50 {runs[-1]['p50_ms']:.0f} ms p99 {runs[-1]['p99_ms']:.0f} ms "
+                         "mean_ms": float(np.mean(lat)), "by_edge": byedge,
+                         "failed": int(sum(1 for x in served if x[0] == li and x[6] != 0))})
+            log(f"[bench] serving load {f:.2f}: p50 {runs[-1]['p50_ms']:.0f} ms p99 {runs[-1]['p99_ms']:.0f} ms "
                 f"{runs[-1]['achieved_solves_per_s']:.1f} solves/s")
         out["loads"] = runs
+    if check_tau:
+        # the FP64 reference of every (volume, direction) pair served, then each request's error
+        conv = {}
+        log("[bench] serving: FP64 reference solves of the pairs served")
+        for key in sorted({(x[1], x[2], x[3]) for x in served}):
+            vol = ctx.volume(pool[key[:2]])
+            res, _ = vol.solve(T.make_params(key[2], CG, "double", 1e-11, 60000))
+            vol.close()
+            conv[key] = res.tau
+        errs = np.array([abs(x[5] - conv[x[1:4]]) / conv[x[1:4]] for x in served])
+        for li, run in enumerate(out["loads"]):
+            e = np.array([errs[k] for k, x in enumerate(served) if x[0] == li])
+            run["tau_rel_err_max"] = float(e.max())
+            run["tau_rel_err_p99"] = float(np.percentile(e, 99))
+            run["tau_within_1e-5"] = int((e <= 1e-5).sum())
+        out["tau_reference"] = f"FP64 CG to 1e-11 of each of the {len(conv)} (volume, direction) pairs served"
+        out["tau_rel_err_max"] = float(errs.max())
+        log(f"[bench] serving: max tau rel err {errs.max():.2e} over {len(served)} requests")
     return out
 
 
@@ -198,7 +248,13 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-iters", type=int, default=25)
-    ap.add_argument("--serve", type=int, default=120, help="requests per offered load (0: skip serving)")
+    ap.add_argument("--serve", type=int, default=200, help="requests per offered load (0: skip serving)")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run only the serving workload and print its JSON (with --serve, --serve-loads, --serve-rate)")
+    ap.add_argument("--serve-loads", default="0.5,0.8,1.1", help="offered loads, fractions of the saturated rate")
+    ap.add_argument("--serve-rate", type=float, default=0.0,
+                    help="saturated rate (solves/s) the loads refer to; 0 measures it with a burst")
+    ap.add_argument("--serve-pool", type=int, default=8, help="volumes per edge length in the request pool")
     ap.add_argument("--no-dram-check", dest="dram_check", action="store_false",
                     help="skip the red-black pass timings on the 512^3 needle volume")
     ap.add_argument("--profile", choices=["solves", "passes"],
@@ -213,6 +269,15 @@ def main() -> int:
         return 2
     if args.profile:
         return profile_program(args.profile, args.size)
+    loads = tuple(float(f) for f in args.serve_loads.split(","))
+    if args.serve_only:
+        from fast_amd import tau as T
+        from fast_amd import workloads as W
+        ctx = T.Context(0)
+        serve = serving(ctx, T, W, args.serve, loads, args.serve_pool, args.serve_rate)
+        print(json.dumps({"serving": serve}), flush=True)
+        ctx.close()
+        return 0
 
     import fast_amd as F
     from fast_amd import tau as T
@@ -329,7 +394,7 @@ def main() -> int:
     serve = None
     if args.serve > 0:
         vol.close()
-        serve = serving(ctx, T, W, args.serve)
+        serve = serving(ctx, T, W, args.serve, loads, args.serve_pool, args.serve_rate)
 
     best_fp64 = max(fp64_rate.values())
     line = {

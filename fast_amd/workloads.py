@@ -70,18 +70,24 @@ REQUEST_EDGES = (64, 128, 192, 256, 384)
 REQUEST_WEIGHTS = (0.3, 0.3, 0.2, 0.15, 0.05)
 
 
-def request_pool(per_edge: int = 4, base_seed: int = 5000) -> dict:
-    """Volumes the request stream draws from: `per_edge` per edge length, ellipsoids with r
-    log-uniform on [0.1, 10] and solid fraction uniform on [0.5, 0.75]."""
-    pool = {}
-    for e in REQUEST_EDGES:
-        for j in range(per_edge):
-            seed = base_seed + 100 * e + j
-            rng = np.random.default_rng(seed)
-            r = _draw_r(rng)
-            solid = float(rng.uniform(0.5, 0.75))
-            pool[(e, j)] = generate_ellipsoids(e, A_SEMI, r, solid, seed)
-    return pool
+def request_volume(edge: int, j: int, base_seed: int = 5000) -> BatchVolume:
+    """Volume j of edge length `edge` in the request pool: an edge^3 ellipsoid volume with r
+    log-uniform on [0.1, 10] and solid fraction uniform on [0.5, 0.75], seeded by (edge, j)."""
+    seed = base_seed + 100 * edge + j
+    rng = np.random.default_rng(seed)
+    r = _draw_r(rng)
+    solid = float(rng.uniform(0.5, 0.75))
+    return BatchVolume(generate_ellipsoids(edge, A_SEMI, r, solid, seed), r, solid, seed)
+
+
+def request_pool(per_edge: int = 4, base_seed: int = 5000, threads: int = 8) -> dict:
+    """Volumes the request stream draws from: `per_edge` per edge length (request_volume),
+    generated on `threads` host threads (the generator releases the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+    keys = [(e, j) for e in REQUEST_EDGES for j in range(per_edge)]
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        vols = list(ex.map(lambda k: request_volume(k[0], k[1], base_seed).mask, keys))
+    return dict(zip(keys, vols))
 
 
 def request_stream(count: int, rate: float, per_edge: int = 4, seed: int = 7) -> list:

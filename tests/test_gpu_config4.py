@@ -1,7 +1,13 @@
 """96 uneven volumes (edges 32..192 per axis, r log-uniform on [0.1, 10], solid 0.5..0.75)
-solved as one batch.  Per-volume tau of the mixed form against the converged FP64 tau (CG to
-1e-11; the FP64 path is bit-identical to the oracle's recurrences), and the FP64 batch
-against solo solves."""
+solved as one batch, against the CPU oracle's FP64 results committed in
+tests/golden/config4_oracle.json (tests/golden/make_oracle_fixtures.py config4; each record
+carries the mask's SHA-256): the converged tau of every volume (CG to 1e-11), which the GPU's
+FP64 batch reproduces bit for bit, and FP64 CG / BiCGSTAB at 1e-6 on four volumes (tau and
+iterations bit-identical).  The reference's CPU path is VolumeMeasures.cpp:42-285."""
+import hashlib
+import json
+from pathlib import Path
+
 import numpy as np
 import pytest
 
@@ -10,24 +16,46 @@ from fast_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
 BICG, CG = 0, 1
+FIXTURE = json.loads((Path(__file__).resolve().parent / "golden" / "config4_oracle.json").read_text())
 
 
 @pytest.fixture(scope="module")
 def batch96(ctx):
     items = W.uneven_batch(96, 1000)
+    for i, b in enumerate(items):
+        assert hashlib.sha256(b.mask.tobytes()).hexdigest() == FIXTURE["converged"][str(i)]["sha256"], i
     vols = [ctx.volume(b.mask) for b in items]
     dirs = [(0, 2, 4)[i % 3] for i in range(len(vols))]
-    print("config4: converged FP64 batch", flush=True)
-    conv = T.solve_batch(ctx, vols, [T.make_params(d, CG, "double", 1e-11, 60000) for d in dirs])
-    for i, r in enumerate(conv):
-        assert r.status == 0, (i, items[i].mask.shape, r.iterations)
+    conv = [FIXTURE["converged"][str(i)] for i in range(len(vols))]
     yield items, vols, dirs, conv
     for v in vols:
         v.close()
 
 
+def test_fp64_batch_equals_the_oracle_converged_tau(ctx, batch96):
+    """FP64 CG to 1e-11 on all 96 volumes in one batch: every tau and iteration count equals
+    the oracle's, bit for bit."""
+    items, vols, dirs, conv = batch96
+    print("config4: FP64 CG 1e-11 batch", flush=True)
+    out = T.solve_batch(ctx, vols, [T.make_params(d, CG, "double", 1e-11, 60000) for d in dirs])
+    for i, (r, c) in enumerate(zip(out, conv)):
+        assert r.status == 0 and c["status"] == 0, i
+        assert r.tau == c["tau"] and r.iterations == c["iterations"], (i, r.tau, c["tau"], r.iterations, c["iterations"])
+    print(f"FP64 CG batch: 96/96 equal the oracle, batch {out[0].solve_ms:.0f} ms")
+
+
+@pytest.mark.parametrize("key", sorted(FIXTURE["fp64"]))
+def test_fp64_solo_equals_the_oracle(ctx, batch96, key):
+    items, vols, dirs, _ = batch96
+    i, solver = int(key.split("_")[0]), (BICG if key.endswith("BICGSTAB") else CG)
+    ref = FIXTURE["fp64"][key]
+    res, _ = vols[i].solve(T.make_params(dirs[i], solver, "double", ref["tol"], ref["max_iter"]))
+    assert res.status == ref["status"] == 0
+    assert res.tau == ref["tau"] and res.iterations == ref["iterations"], key
+
+
 def _errs(out, conv):
-    return np.array([abs(r.tau - c.tau) / c.tau for r, c in zip(out, conv)])
+    return np.array([abs(r.tau - c["tau"]) / c["tau"] for r, c in zip(out, conv)])
 
 
 def test_mixed_cg_batch_within_1e5_of_converged(ctx, batch96):
@@ -42,8 +70,7 @@ def test_mixed_cg_batch_within_1e5_of_converged(ctx, batch96):
     worst = int(np.argmax(errs))
     print(f"mixed CG: max rel err {errs.max():.2e} (volume {worst}, shape {items[worst].mask.shape}, "
           f"r={items[worst].r:.2f}), median {np.median(errs):.2e}, iterations "
-          f"{min(r.iterations for r in out)}..{max(r.iterations for r in out)}, batch {out[0].solve_ms:.0f} ms "
-          f"(FP64 CG 1e-11 batch {conv[0].solve_ms:.0f} ms)")
+          f"{min(r.iterations for r in out)}..{max(r.iterations for r in out)}, batch {out[0].solve_ms:.0f} ms")
     assert errs.max() <= 1e-5
 
 
