@@ -119,6 +119,7 @@ PROTOTYPES = [
     ("fast_ccl_segment_info", C.c_int, [_vp, C.c_uint8, C.POINTER(SegmentInfo), C.c_uint32]),
     ("fast_alpha_area", C.c_int, [_vp, C.c_float, C.POINTER(C.c_double)]),
     ("fast_alpha_area_noalloc", C.c_int, [_vp, C.c_float, C.POINTER(C.c_double)]),
+    ("fast_debug_stream", C.c_int, [_vp, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
     ("fast_alpha_mesh", C.c_int, [_vp, C.c_float, C.POINTER(C.c_float), C.c_size_t, C.POINTER(C.c_size_t)]),
     ("fast_alpha_save_obj", C.c_int, [_vp, C.c_float, C.c_char_p, C.POINTER(C.c_size_t)]),
     ("fast_alpha_area_tensor", C.c_int, [_vp, C.c_float, C.POINTER(C.c_double)]),

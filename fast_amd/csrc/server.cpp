@@ -39,7 +39,8 @@ double now_s() {
 
 constexpr int kTX = 64, kTY = 8;      // one launch shape for the whole live set (full grid)
 constexpr size_t kAlign = 64;         // arena granularity (elements)
-constexpr size_t kPlanes = 8;         // full grid: FP32 vectors x (the correction), r, r0, p[2], v[2], s
+constexpr size_t kPlanes = 4;         // full grid: FP32 vectors x (the correction), r, r0, s
+constexpr size_t kDirPlanes = 4;      // full grid: FP64 directions p[2], v[2] (BiCGSTAB's, see DirOf)
 // red-black: per padded per-colour element, FP32 x, r, p[2], q, FP64 y, x64, x64b and two code bytes
 constexpr size_t kRbBytes = 5 * sizeof(float) + 3 * sizeof(double) + 2;
 
@@ -205,6 +206,7 @@ VolState<double> fresh_state(const fast_server* s, const Req* r) {
     const double delta = (r->prm.mixed_inner_tol > 0 && r->prm.mixed_inner_tol < 1) ? r->prm.mixed_inner_tol : 0.1;
     v.delta2 = delta * delta;
     v.restart_cos = s->ctx->mixed_restart_cos;
+    v.blowup2 = s->ctx->mixed_blowup2;
     v.eps2 = s->rb ? 2.220446049250313e-16 * 2.220446049250313e-16 : 1.1920928955078125e-07 * 1.1920928955078125e-07;
     return v;
 }
@@ -267,7 +269,7 @@ int rebuild_rb(fast_server* s, const std::vector<Req*>& adm) {
     RbLaunch& L = s->RL;
     L.ntiles = tiles;
     L.nvol = s->max_active;
-    L.staged = 1;
+    L.staged = rb_mode();
     L.vol = s->d_rb;
     L.tile_vol = s->d_tile;
     L.coef = s->d_coef;
@@ -362,12 +364,13 @@ int rebuild(fast_server* s, const std::vector<Req*>& adm) {
     L.vs.x = a + 0 * S;
     L.vs.r = a + 1 * S;
     L.vs.r0 = a + 2 * S;
-    L.vs.p[0] = a + 3 * S;
-    L.vs.p[1] = a + 4 * S;
-    L.vs.v[0] = a + 5 * S;
-    L.vs.v[1] = a + 6 * S;
-    L.vs.s = a + 7 * S;
-    L.vs.x64 = (double*)(a + 8 * S);
+    L.vs.s = a + 3 * S;
+    double* d = (double*)(a + kPlanes * S);
+    L.vs.p[0] = d + 0 * S;
+    L.vs.p[1] = d + 1 * S;
+    L.vs.v[0] = d + 2 * S;
+    L.vs.v[1] = d + 3 * S;
+    L.vs.x64 = d + kDirPlanes * S;
     if (!adm.empty()) {
         const hipError_t et = ft::build_tables(L, ctx->stream);   // descriptors and coefficients are on the device
         if (!hip_ok(ctx, et, "server tables")) return FAST_E_HIP;
@@ -624,7 +627,7 @@ int fast_server_create(fast_ctx* ctx, const fast_server_config* cfg, fast_server
     const size_t budget = s->cfg.hbm_budget ? s->cfg.hbm_budget : std::min(ctx->budget / 2, (size_t)16 << 30);
     // bytes per arena element: red-black, kRbBytes per padded per-colour element; full grid,
     // 8 FP32 planes + the FP64 solution per voxel
-    const size_t per = s->rb ? kRbBytes : kPlanes * sizeof(float) + sizeof(double);
+    const size_t per = s->rb ? kRbBytes : kPlanes * sizeof(float) + (kDirPlanes + 1) * sizeof(double);
     s->S = align_up(budget / per, kAlign);
     if (s->S * per > budget && s->S > kAlign) s->S -= kAlign;
     s->arena_mem = dalloc(ctx, s->S * per);

@@ -277,6 +277,10 @@ int fast_ctx_create(int device, size_t hbm_budget, fast_ctx** out) {
     }
     load_tuning(ctx);
     if (const char* e = std::getenv("FAST_TAU_MIXED_RESTART")) ctx->mixed_restart_cos = std::atof(e);
+    if (const char* e = std::getenv("FAST_TAU_MIXED_BLOWUP")) {
+        const double g = std::atof(e);   // growth of |r| (not squared); 0 = never on growth alone
+        ctx->mixed_blowup2 = g > 0 ? g * g : HUGE_VAL;
+    }
     if (const char* e = std::getenv("FAST_TAU_GRAPHS")) ctx->use_graphs = std::atoi(e) != 0;
     if (const char* e = std::getenv("FAST_TAU_GROUP_MB")) ctx->group_bytes = (size_t)std::atol(e) << 20;
     if (const char* e = std::getenv("FAST_TAU_UPDATE_EVERY")) ctx->update_every = std::max(1, std::atoi(e));
@@ -517,6 +521,7 @@ static int prepare(Batch<T, A>& B) {
             s.otol = j.outer_tol;
             s.delta2 = j.update_delta * j.update_delta;
             s.restart_cos = ctx->mixed_restart_cos;
+            s.blowup2 = ctx->mixed_blowup2;
             s.solver = j.prm.solver;
         }
     }
@@ -558,8 +563,12 @@ static int prepare(Batch<T, A>& B) {
     o = align_up(o + sizeof(typename Batch<T, A>::C) * 2 * kTabEntries * nv, 256);
     const size_t meta_bytes = o;
     B.meta = dalloc(ctx, meta_bytes);
-    const int nvec = 8;   // x, r, r0, p[2], v[2], s
-    const size_t vec_bytes = nvec * total * sizeof(T) + (B.mixed ? total * sizeof(double) : 0);
+    // x, r, r0, s in T; p[2], v[2] in the directions' type (FP64 for mixed BiCGSTAB, DirOf);
+    // the mixed form's FP64 solution
+    bool bicg = false;
+    for (const Job* j : B.jobs) bicg = bicg || j->prm.solver == FAST_BICGSTAB;
+    const size_t dsz = bicg ? sizeof(typename DirOf<T, A>::type) : sizeof(T);
+    const size_t vec_bytes = 4 * total * sizeof(T) + 4 * total * dsz + (B.mixed ? total * sizeof(double) : 0);
     B.vecs = dalloc(ctx, vec_bytes);
     if (!B.meta || !B.vecs) return FAST_E_OOM;
     std::vector<char> h(meta_bytes, 0);
@@ -588,17 +597,18 @@ static int prepare(Batch<T, A>& B) {
     B.plane_k = (int*)(m + o_pk);
     B.plane_out = (double*)(m + o_pout);
     B.L.tab = (typename Batch<T, A>::C*)(m + o_tab);
-    T* vb = (T*)B.vecs;
+    char* vb = (char*)B.vecs;
+    const size_t tb = total * sizeof(T), db = total * dsz;
     VecSet& vs = B.L.vs;
-    vs.x = vb + 0 * total;
-    vs.r = vb + 1 * total;
-    vs.r0 = vb + 2 * total;
-    vs.p[0] = vb + 3 * total;
-    vs.p[1] = vb + 4 * total;
-    vs.v[0] = vb + 5 * total;
-    vs.v[1] = vb + 6 * total;
-    vs.s = vb + 7 * total;
-    vs.x64 = B.mixed ? (double*)(vb + 8 * total) : nullptr;
+    vs.x = vb;
+    vs.r = vb + tb;
+    vs.r0 = vb + 2 * tb;
+    vs.s = vb + 3 * tb;
+    vs.p[0] = vb + 4 * tb;
+    vs.p[1] = vb + 4 * tb + db;
+    vs.v[0] = vb + 4 * tb + 2 * db;
+    vs.v[1] = vb + 4 * tb + 3 * db;
+    vs.x64 = B.mixed ? (double*)(vb + 4 * tb + 4 * db) : nullptr;
     e = ft::build_tables(B.L, ctx->stream);
     return hip_ok(ctx, e, "diagonal tables") ? FAST_OK : FAST_E_HIP;
 }
@@ -838,6 +848,11 @@ struct RbBatch {
     }
 };
 
+int rb_mode() {
+    const char* e = std::getenv("FAST_TAU_RB_STAGED");
+    return e ? std::min(2, std::max(0, std::atoi(e))) : 2;
+}
+
 // Planes per RB tile: enough tiles to fill the chip (about 2048), 4..32 planes each.
 int rb_tz(int hx, int ny, int nz) {
     if (const char* e = std::getenv("FAST_TAU_RB_TZ")) {
@@ -912,6 +927,7 @@ static int prepare_rb(RbBatch& B) {
         s.otol = j.outer_tol;
         s.delta2 = j.update_delta * j.update_delta;
         s.restart_cos = ctx->mixed_restart_cos;
+        s.blowup2 = ctx->mixed_blowup2;
         s.solver = FAST_CG;
     }
     B.ntiles = tiles;
@@ -977,8 +993,7 @@ static int prepare_rb(RbBatch& B) {
     RbLaunch& L = B.L;
     L.ntiles = tiles;
     L.nvol = nv;
-    const char* stg = std::getenv("FAST_TAU_RB_STAGED");
-    L.staged = !(stg && stg[0] == '0');
+    L.staged = rb_mode();
     L.vol = (const RbDesc*)(m + o_rb);
     L.tile_vol = (const int*)(m + o_tile);
     L.coef = (const Coef<double>*)(m + o_coef);
@@ -1364,6 +1379,40 @@ int fast_debug_time_passes(fast_volume* vol, int dir, int precision, int solver,
 int fast_debug_time_rb_passes(fast_volume* vol, int dir, int iters, double* ms) {
     if (!vol || !ms || iters < 1 || dir < 0 || dir > 5) return FAST_E_BADARG;
     return time_rb_passes(vol, dir, iters, ms);
+}
+
+int fast_debug_stream(fast_ctx* ctx, size_t bytes, int iters, double* ms) {
+    if (!ctx || !ms || iters < 1 || bytes < 16) return FAST_E_BADARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    const size_t n4 = bytes / 16;
+    float4* a = (float4*)dalloc(ctx, n4 * 16);
+    float4* b = (float4*)dalloc(ctx, n4 * 16);
+    float4* c = (float4*)dalloc(ctx, n4 * 16);
+    int rc = (a && b && c) ? FAST_OK : FAST_E_OOM;
+    hipStream_t s = ctx->stream;
+    hipError_t e = hipSuccess;
+    // cases: triad one float4 per thread, triad on 8192 striding blocks, copy, fill
+    const int kinds[4] = {0, 0, 1, 2};
+    const unsigned grids[4] = {0, 8192, 0, 0};
+    if (!rc) e = ft::stream_kernel(2, a, nullptr, nullptr, n4, 0, s);
+    if (e == hipSuccess && !rc) e = ft::stream_kernel(2, b, nullptr, nullptr, n4, 0, s);
+    for (int k = 0; k < 4 && e == hipSuccess && !rc; ++k) {
+        e = ft::stream_kernel(kinds[k], a, b, c, n4, grids[k], s);   // untimed first launch
+        if (e == hipSuccess) e = hipEventRecord(ctx->t0, s);
+        for (int i = 0; i < iters && e == hipSuccess; ++i) e = ft::stream_kernel(kinds[k], a, b, c, n4, grids[k], s);
+        if (e == hipSuccess) e = hipEventRecord(ctx->t1, s);
+        if (e == hipSuccess) e = hipEventSynchronize(ctx->t1);
+        float t = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, ctx->t0, ctx->t1);
+        ms[k] = (double)t / iters;
+    }
+    if (!rc && !hip_ok(ctx, e, "stream timing")) rc = FAST_E_HIP;
+    (void)hipStreamSynchronize(s);
+    dfree(ctx, c);
+    dfree(ctx, b);
+    dfree(ctx, a);
+    return rc;
 }
 
 int fast_debug_apply_f64(fast_volume* vol, int dir, const double* x, double* y) { return debug_apply<double>(vol, dir, x, y); }

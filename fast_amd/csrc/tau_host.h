@@ -3,6 +3,7 @@
 #include "fast_tau.h"
 #include "tau_internal.h"
 
+#include <cmath>
 #include <map>
 #include <mutex>
 #include <string>
@@ -51,7 +52,11 @@ struct fast_ctx {
     bool use_graphs = true;                    // replay polling rounds from a captured HIP graph
     size_t group_bytes = (size_t)320 << 20;    // red-black batches: state bytes per sub-batch (0 = one launch)
     int update_every = fth::kMixedUpdateEvery; // mixed form: iterations between reliable-update steps
-    double mixed_restart_cos = 1e-6;           // mixed form: restart when r0 and r are this close to orthogonal
+    // mixed BiCGSTAB: reset r0 = r when r0 and r are this close to orthogonal (0: only by
+    // Eigen's rule), and drop the correction when |r|^2 grows this much since the last update
+    // (infinite: only a non-finite residual does)
+    double mixed_restart_cos = 0.0;
+    double mixed_blowup2 = HUGE_VAL;
     std::unordered_map<fast_volume*, int> volumes;   // live volumes, released with the context
     std::mutex mu;
 };
@@ -98,5 +103,8 @@ void load_tuning(fast_ctx* ctx);
 int solve_jobs(fast_ctx* ctx, std::vector<Job>& jobs);
 // Planes per tile of the red-black passes for a volume with hx slots per row.
 int rb_tz(int hx, int ny, int nz);
+// Form of the red-black Y and Q passes (RbLaunch::staged): FAST_TAU_RB_STAGED = 0 (separate,
+// Q reads six Y values), 1 (separate, Q stages the Y planes in LDS), default 2 (fused).
+int rb_mode();
 
 }  // namespace fth

@@ -46,7 +46,9 @@ struct VolState {
     // mixed form (FP32 Krylov process with FP64 reliable updates)
     double r64sq, b64sq, otol;     // last FP64 |r|^2, |b|^2, relative tolerance on the true residual
     double rsq_ref, delta2;        // |r|^2 at the last update; update when |r|^2 < delta2 * rsq_ref
-    double restart_cos;            // mixed: fresh start when |r0.r| < restart_cos |r0| |r|
+    double restart_cos;            // mixed BiCGSTAB: reset r0 = r when |r0.r| < restart_cos |r0| |r|
+    double blowup2;                // mixed BiCGSTAB: drop the correction when |r|^2 > blowup2 * rsq_ref
+    double rho_rec;                // mixed BiCGSTAB: r0.r of the recursive residual at the update
     long long maxiter;             // Krylov iteration cap
     int iter, status, restarts, first;
     int phase;                     // ping-pong parity counter (never reset)
@@ -80,6 +82,19 @@ struct CmpOf {
 };
 template <>
 struct CmpOf<float, double> {
+    using type = double;
+};
+
+// Storage type of BiCGSTAB's directions p and v.  In the mixed form they stay FP64: on
+// ill-conditioned volumes BiCGSTAB's p and v grow to 1e5..1e7 times the residual between
+// updates, and rounding them to FP32 (relative error 6e-8 of their own size) would perturb
+// the residual by order one; the residual, shadow residual, s and the correction stay FP32.
+template <class T, class A>
+struct DirOf {
+    using type = T;
+};
+template <>
+struct DirOf<float, double> {
     using type = double;
 };
 
@@ -169,7 +184,8 @@ struct RbDesc {
 
 struct RbLaunch {
     int ntiles, nvol;
-    int staged;       // 1: Q pass with the Y planes staged in LDS; 0: six Y loads per voxel
+    int staged;       // 2: Y and Q fused (no Y vector); 1: Q pass with the Y planes staged in LDS; 0: six
+                      // Y loads per voxel
     const RbDesc* vol;
     const int* tile_vol;
     const Coef<double>* coef;
@@ -196,6 +212,9 @@ hipError_t rb_iter(const RbLaunch&, hipStream_t);
 // Reliable update / start for the volumes in ST_UPDATE or ST_FRESH: x64 += x, x = 0, then the
 // FP64 residual r = b_S - S x64 (x64 = x0 on the first step) and the FP64 convergence test.
 hipError_t rb_update(const RbLaunch&, hipStream_t);
+// Streaming kernels for the measured HBM ceiling: kind 0 c = a + s b, 1 b = a, 2 a = s, over
+// n4 float4 per buffer; blocks = 0 launches one float4 per thread, else a striding grid.
+hipError_t stream_kernel(int kind, float4* a, float4* b, float4* c, size_t n4, unsigned blocks, hipStream_t s);
 // One pass of rb_iter alone (0 = P, 1 = Y, 2 = Q, 3 = U), for timing.
 hipError_t rb_pass_only(const RbLaunch&, int k, hipStream_t);
 // Both colours scattered into the full-grid solution (the last update left x64b = x_b(x64)).

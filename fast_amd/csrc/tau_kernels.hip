@@ -335,6 +335,20 @@ struct PassCtx {
     T* v_old;
     T* v_new;
     T* s;       // BiCGSTAB's s = r - alpha v (pass 2 -> pass 3); the apply diagnostic's output
+    // BiCGSTAB's p and v in their own storage type (the same buffers as p_old .. v_new)
+    using D = typename DirOf<T, A>::type;
+    D* bp_old;
+    D* bp_new;
+    D* bv_old;
+    D* bv_new;
+};
+
+// Raw loads of the BiCGSTAB passes: residual-side vectors in T, directions in D.
+template <class T, class D>
+struct RawB {
+    T a, e, f;
+    D b, c;
+    int code;
 };
 
 template <class T, class A, int KIND>
@@ -344,6 +358,7 @@ struct PassOps;
 template <class T, class A>
 struct PassOps<T, A, K_INIT_BICG> {
     using C = typename CmpOf<T, A>::type;
+    using D = typename DirOf<T, A>::type;
     using R = Raw<T>;   // raw loads in the storage type; widened in compute()
     static constexpr int NP = 2;   // |r|^2, |b|^2
     template <class X>
@@ -367,8 +382,8 @@ struct PassOps<T, A, K_INIT_BICG> {
         st(pc.x, i, T(pt.u));
         st(pc.r, i, T(r));
         st(pc.r0, i, T(r));
-        st(pc.p_old, i, T(0));
-        st(pc.v_old, i, T(0));
+        st(pc.bp_old, i, D(0));
+        st(pc.bv_old, i, D(0));
         acc[0].add(r, r);
         acc[1].add(b, b);
     }
@@ -473,14 +488,15 @@ struct PassOps<T, A, K_APPLY> {
 template <class T, class A>
 struct PassOps<T, A, K_BICG1> {
     using C = typename CmpOf<T, A>::type;
-    using R = Raw<T>;   // raw loads in the storage type; widened in compute()
+    using D = typename DirOf<T, A>::type;
+    using R = RawB<T, D>;   // raw loads in the storage types; widened in compute()
     static constexpr int NP = 1;
     template <class X>
     __device__ static R fetch(const X& pc, unsigned i, bool centre) {
         R w;
         w.a = ld(pc.r, i);
-        w.b = ld(pc.p_old, i);
-        w.c = ld(pc.v_old, i);
+        w.b = ld(pc.bp_old, i);
+        w.c = ld(pc.bv_old, i);
         if (centre) w.e = ld(pc.r0, i);
         w.code = ld(pc.code, i);
         return w;
@@ -490,7 +506,7 @@ struct PassOps<T, A, K_BICG1> {
         Pt<C> q;
         q.code = w.code;
         // p as it will be stored: the stencil and the x update must see the same direction
-        const C p = C(T(C(w.a) + pc.beta * (C(w.b) - pc.omega * C(w.c))));
+        const C p = C(D(C(w.a) + pc.beta * (C(w.b) - pc.omega * C(w.c))));
         {
             const DiagInv<C> di = diag_at<C>(pc.tab, w.code, bx, pc.primary, pc.c);
             q.dg = di.d;
@@ -503,8 +519,8 @@ struct PassOps<T, A, K_BICG1> {
     }
     template <class X, class Acc>
     __device__ static void epi(const X& pc, unsigned i, int k, C Au, const Pt<C>& pt, Acc (&acc)[NP]) {
-        st(pc.p_new, i, T(pt.aux));
-        st(pc.v_new, i, T(Au));
+        st(pc.bp_new, i, D(pt.aux));
+        st(pc.bv_new, i, D(Au));
         acc[0].add(pt.e, Au);
     }
 };
@@ -515,13 +531,14 @@ struct PassOps<T, A, K_BICG1> {
 template <class T, class A>
 struct PassOps<T, A, K_BICG2> {
     using C = typename CmpOf<T, A>::type;
-    using R = Raw<T>;   // raw loads in the storage type; widened in compute()
+    using D = typename DirOf<T, A>::type;
+    using R = RawB<T, D>;   // raw loads in the storage types; widened in compute()
     static constexpr int NP = 2;
     template <class X>
     __device__ static R fetch(const X& pc, unsigned i, bool centre) {
         R w;
         w.a = ld(pc.r, i);
-        w.b = ld(pc.v_new, i);
+        w.b = ld(pc.bv_new, i);
         w.code = ld(pc.code, i);
         return w;
     }
@@ -553,14 +570,15 @@ struct PassOps<T, A, K_BICG2> {
 template <class T, class A>
 struct PassOps<T, A, K_BICG3> {
     using C = typename CmpOf<T, A>::type;
-    using R = Raw<T>;   // raw loads in the storage type; widened in compute()
+    using D = typename DirOf<T, A>::type;
+    using R = RawB<T, D>;   // raw loads in the storage types; widened in compute()
     static constexpr int NP = 2;
     template <class X>
     __device__ static R fetch(const X& pc, unsigned i, bool centre) {
         R w;
         w.a = ld(pc.s, i);
         if (centre) {
-            w.c = ld(pc.p_new, i);
+            w.c = ld(pc.bp_new, i);
             w.e = ld(pc.r0, i);
             w.f = ld(pc.x, i);
         }
@@ -864,6 +882,11 @@ __global__ __launch_bounds__(TX* TY) void k_stencil(Launch<T, A> L, const T* xin
         pc.p_new = (T*)(odd ? L.vs.p[0] : L.vs.p[1]) + o;
         pc.v_old = (T*)(odd ? L.vs.v[1] : L.vs.v[0]) + o;
         pc.v_new = (T*)(odd ? L.vs.v[0] : L.vs.v[1]) + o;
+        using D = typename DirOf<T, A>::type;
+        pc.bp_old = (D*)(odd ? L.vs.p[1] : L.vs.p[0]) + o;
+        pc.bp_new = (D*)(odd ? L.vs.p[0] : L.vs.p[1]) + o;
+        pc.bv_old = (D*)(odd ? L.vs.v[1] : L.vs.v[0]) + o;
+        pc.bv_new = (D*)(odd ? L.vs.v[0] : L.vs.v[1]) + o;
         pc.s = (T*)L.vs.s + o;
         if (KIND == K_APPLY) {
             pc.x = (T*)xin + o;
@@ -935,22 +958,25 @@ __device__ __forceinline__ void bicg_loop_head(VolState<A>* s, A rr0) {
     s->status = ST_RUNNING;
 }
 
-// Mixed form, end of an iteration.  Reliable update (x64 += d, residual replaced by the FP64
-// true residual) when the recursive residual has dropped by delta since the last one, when it
-// meets the target (the true residual decides) or at the iteration cap.  With FP32-stored
-// vectors r0.r loses its accuracy once r0 and r are nearly orthogonal, so below
-// restart_cos the update also resets the shadow residual r0 = r (Eigen's restart rule,
-// which keeps p and v).  A blow-up (residual up 100x since the last update or non-finite)
-// drops the correction and starts afresh from x64.
+// Mixed form, end of a BiCGSTAB iteration.  Reliable update (x64 += d, residual replaced by
+// the FP64 true residual) when the recursive residual has dropped by delta since the last one,
+// when it meets the target (the true residual decides) or at the iteration cap; the process
+// then continues with p, v and the recursive r0.r (rho_rec), which are consistent with each
+// other, rather than with r0 dotted with the replaced residual.  Eigen's rule (|r0.r| below
+// eps^2 |r0|^2) and, when set, restart_cos also reset the shadow residual r0 = r at the
+// update.  A non-finite residual, or one grown by blowup2 since the last update, drops the
+// correction and starts afresh from x64.  BiCGSTAB's residual peaks by orders of magnitude
+// on ill-conditioned volumes and recovers, so growth alone drops nothing by default.
 template <class A>
 __device__ __forceinline__ void mixed_after_iteration(VolState<A>* s, A rr0) {
-    if (!isfinite(s->rsq) || s->rsq > 1e4 * s->rsq_ref) {
+    if (!isfinite(s->rsq) || s->rsq > s->blowup2 * s->rsq_ref) {
         s->status = ST_FRESH;
         s->discard = 1;
         return;
     }
     if (!(s->rsq > s->tol2) || s->rsq < s->delta2 * s->rsq_ref || s->iter >= s->maxiter) {
         s->status = ST_UPDATE;
+        s->rho_rec = rr0;
         return;
     }
     if (!isfinite(rr0) || fabs(rr0) < s->eps2 * s->r0sq || fabs(rr0) < s->restart_cos * sqrt(s->r0sq * s->rsq)) {
@@ -1367,8 +1393,8 @@ struct MixCtx {
     double* x64;
     float* r;
     float* r0;
-    float* p_cur;
-    float* v_cur;
+    double* p_cur;   // BiCGSTAB's directions (FP64 in the mixed form)
+    double* v_cur;
 };
 
 // FP64 true residual of the mixed form: u = x64 (x0 on the first step); r64 = b - A x64
@@ -1411,9 +1437,9 @@ struct MixOps {
         if (pc.first) st(pc.x64, i, pt.u);
         st(pc.r, i, r32);
         if (pc.reshadow) st(pc.r0, i, r32);
-        if (pc.fresh) {
-            st(pc.p_cur, i, 0.0f);
-            st(pc.v_cur, i, 0.0f);
+        if (pc.fresh && pc.solver == 0) {   // CG starts afresh with p = z (first), reading no p
+            st(pc.p_cur, i, 0.0);
+            st(pc.v_cur, i, 0.0);
         }
         acc[0].add(r64, r64);
         acc[1].add(b, b);
@@ -1455,8 +1481,8 @@ __global__ __launch_bounds__(TX* TY) void k_mixres(Launch<float, double> L, cons
     pc.x64 = L.vs.x64 + o;
     pc.r = (float*)L.vs.r + o;
     pc.r0 = (float*)L.vs.r0 + o;
-    pc.p_cur = (float*)(odd ? L.vs.p[1] : L.vs.p[0]) + o;
-    pc.v_cur = (float*)(odd ? L.vs.v[1] : L.vs.v[0]) + o;
+    pc.p_cur = (double*)(odd ? L.vs.p[1] : L.vs.p[0]) + o;
+    pc.v_cur = (double*)(odd ? L.vs.v[1] : L.vs.v[0]) + o;
     load_tables<double, TX * TY>(L.tab + (long long)v * 2 * kTabEntries, tab, ftab, pc.c);
     AccDD acc[4];
 #pragma unroll
@@ -1519,7 +1545,7 @@ __device__ void mixres_update(VolState<double>* s, const double* tot) {
     }
     if (status == ST_FRESH || reshadow) s->r0sq = tot[2];
     if (reshadow && status != ST_FRESH) s->restarts += 1;
-    const double rho_new = tot[3];
+    const double rho_new = (status != ST_FRESH && !reshadow && isfinite(s->rho_rec)) ? s->rho_rec : tot[3];
     if (status != ST_FRESH && !reshadow &&
         (!isfinite(rho_new) || fabs(rho_new) < s->restart_cos * sqrt(s->r0sq * s->rsq))) {
         s->status = ST_UPDATE;   // the replaced residual is too orthogonal to r0: reset r0 next
@@ -1938,6 +1964,200 @@ __global__ __launch_bounds__(kRbTX* kRbTY) void k_rb_q(RbLaunch L) {
     block_reduce_store<AccDD, 1, NT>(acc, L.part + (long long)tile * kPart);
 }
 
+// Y and Q fused: q = S p = D_r p - A_rb (D_b^-1 A_br p) for a block's red tile without
+// storing Y.  A block owns kRbTX slots x kRbTY rows of red voxels and marches z.  It keeps red
+// p planes over the tile plus a two-slot, two-row halo in a four-plane LDS ring, computes the
+// black Y of planes over the tile plus a one-slot, one-row halo from them into a second
+// four-plane ring, and forms q from that.  Per red voxel the pass reads p and the two codes
+// and writes q (10 bytes instead of the 30 of the separate Y and Q passes: Y is FP64, written
+// once and read back); the halo Y values are computed twice by neighbouring blocks instead.
+// Step z computes Y(z + 1) from p(z .. z + 2), stores p(z + 3) over p(z - 1), passes one
+// barrier and forms q(z) from Y(z - 1 .. z + 1): the slot written into holds a plane that was
+// last read before the previous step's barrier.  Y and q take the same values, roundings and
+// summation order as k_rb_it<RB_Y> followed by k_rb_it<RB_Q>, so results are bit-identical.
+__global__ __launch_bounds__(kRbTX* kRbTY) void k_rb_yq(RbLaunch L) {
+    constexpr int NT = kRbTX * kRbTY;
+    constexpr int PW = kRbTX + 4, PPL = PW * (kRbTY + 4);   // p plane: slots k0-2 .. k0+65, rows y0-2 .. y0+5
+    constexpr int YW = kRbTX + 2, YPL = YW * (kRbTY + 2);   // Y plane: slots k0-1 .. k0+64, rows y0-1 .. y0+4
+    constexpr int PR = (PPL + NT - 1) / NT;
+    constexpr int YR = (YPL + NT - 1) / NT;
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
+    if (tile >= L.ntiles) return;
+    const int v = L.tile_vol[tile];
+    const RbDesc d = L.vol[v];
+    const VolState<double>* vs = L.st + v;
+    if (vs->status != ST_RUNNING) return;
+    __shared__ RbTabs tabs;
+    __shared__ float pring[4 * PPL];
+    __shared__ double yring[4 * YPL];
+    const Coef<double> c = L.coef[v];
+    rb_load_tables<2>(L, v, c, tabs);   // Y reads D_b^-1, Q reads D_r
+    const int lt = tile - d.tile_base;
+    const int k0 = (lt % d.tiles_x) * kRbTX;
+    const int y0 = ((lt / d.tiles_x) % d.tiles_y) * kRbTY;
+    const int z0 = (lt / (d.tiles_x * d.tiles_y)) * d.tz;
+    const int z1 = min(z0 + d.tz, d.nz);
+    const bool odd = (vs->phase & 1) != 0;
+    const float* P = L.p[odd ? 0 : 1] + d.off_c;
+    const uint8_t* code_r = d.code_c[0];
+    const uint8_t* code_b = d.code_c[1];
+    const int tid = (int)threadIdx.x;
+    // p halo points of this thread: element within a plane, or ~0u outside the padded layout
+    // (such points feed no Y that is computed)
+    unsigned pofs[PR];
+#pragma unroll
+    for (int j = 0; j < PR; ++j) {
+        const int i = tid + j * NT;
+        const int kk = k0 - 2 + i % PW, yy = y0 - 2 + i / PW;
+        pofs[j] = (i < PPL && kk >= -1 && kk <= d.hx && yy >= -1 && yy <= d.ny)
+                      ? (unsigned)(kk + 1) + (unsigned)d.px * (unsigned)(yy + 1) : ~0u;
+    }
+    // element 0 of the padded layout is a ghost (+0): loads outside the layout read it
+    auto p_load = [&](int zc, float (&h)[PR]) {
+#pragma unroll
+        for (int j = 0; j < PR; ++j) {
+            const bool ok = pofs[j] != ~0u && zc >= -1 && zc <= d.nz;
+            h[j] = ld(P, ok ? pofs[j] + d.pp * (unsigned)(zc + 1) : 0u);
+        }
+    };
+    auto p_store = [&](int zc, const float (&h)[PR]) {
+        float* dst = pring + (zc & 3) * PPL;
+#pragma unroll
+        for (int j = 0; j < PR; ++j) {
+            const int i = tid + j * NT;
+            if (i < PPL) dst[i] = h[j];
+        }
+    };
+    // black Y points of this thread: slot kb, row yb of the Y plane
+    int ykb[YR], yyb[YR];
+#pragma unroll
+    for (int j = 0; j < YR; ++j) {
+        const int i = tid + j * NT;
+        ykb[j] = k0 - 1 + i % YW;
+        yyb[j] = y0 - 1 + i / YW;
+    }
+    auto y_valid = [&](int j, int zb, int& s, int& x) {
+        const int i = tid + j * NT;
+        s = (1 + yyb[j] + zb) & 1;
+        x = 2 * ykb[j] + s;
+        return i < YPL && ykb[j] >= 0 && yyb[j] >= 0 && yyb[j] < d.ny && zb >= 0 && zb < d.nz && x < d.nx;
+    };
+    auto yc_load = [&](int zb, int (&cb)[YR]) {
+#pragma unroll
+        for (int j = 0; j < YR; ++j) {
+            int s, x;
+            cb[j] = ld(code_b, y_valid(j, zb, s, x) ? rb_index(d, ykb[j], yyb[j], zb) : 0u);
+        }
+    };
+    auto y_plane = [&](int zb, const int (&cb)[YR]) {
+        const float* pm = pring + ((zb - 1) & 3) * PPL;
+        const float* p0 = pring + (zb & 3) * PPL;
+        const float* pp1 = pring + ((zb + 1) & 3) * PPL;
+        double* dst = yring + (zb & 3) * YPL;
+#pragma unroll
+        for (int j = 0; j < YR; ++j) {
+            const int i = tid + j * NT;
+            if (i >= YPL) continue;
+            int s, x;
+            double yv = 0.0;   // ghosts and slots past the row end hold +0, as in the Y vector
+            if (y_valid(j, zb, s, x)) {
+                const int pc = (ykb[j] - k0 + 2) + (yyb[j] - y0 + 2) * PW;
+                float u[6];
+                u[0] = pm[pc];
+                u[1] = p0[pc - PW];
+                u[2] = p0[pc - 1 + s];
+                u[3] = p0[pc + s];
+                u[4] = p0[pc + PW];
+                u[5] = pp1[pc];
+                const int bx = box_flags(x, yyb[j], zb, d.nx, d.ny, d.nz);
+                const DiagInv<double> di = diag_at<double>(tabs.tab, cb[j], bx, d.primary, c);
+                yv = rb_sum(tabs.ftab, cb[j], u) * di.iv;
+            }
+            dst[i] = yv;
+        }
+    };
+    const int k = k0 + (tid % kRbTX), y = y0 + (tid / kRbTX);
+    const bool row_ok = k < d.hx && y < d.ny;
+    const int ycol = (k - k0 + 1) + (y - y0 + 1) * YW;   // this thread's slot in a Y plane
+    const int pcol = (k - k0 + 2) + (y - y0 + 2) * PW;   // and in a p plane
+    auto q_valid = [&](int z, int& s, int& x) {
+        s = (y + z) & 1;
+        x = 2 * k + s;
+        return row_ok && z < z1 && x < d.nx;
+    };
+    auto rc_load = [&](int z) {
+        int s, x;
+        return (int)ld(code_r, q_valid(z, s, x) ? rb_index(d, k, y, z) : 0u);
+    };
+    AccDD acc[1];
+    acc[0].init();
+    // p of this thread's red voxel in plane z, read before the step's barrier: the slot is
+    // refilled with p(z + 4) in the next step, possibly while a slower thread forms q(z)
+    auto p_own = [&](int z) { return pring[(z & 3) * PPL + pcol]; };
+    auto q_plane = [&](int z, int code, float pv) {
+        int s, x;
+        if (!q_valid(z, s, x)) return;
+        const int bx = box_flags(x, y, z, d.nx, d.ny, d.nz);
+        const DiagInv<double> di = diag_at<double>(tabs.tab, code, bx, d.primary, c);
+        const double* pl = yring + (z & 3) * YPL + ycol;
+        double o[6];
+        o[0] = yring[((z - 1) & 3) * YPL + ycol];
+        o[1] = pl[-YW];
+        o[2] = pl[s - 1];
+        o[3] = pl[s];
+        o[4] = pl[YW];
+        o[5] = yring[((z + 1) & 3) * YPL + ycol];
+        const double p64 = (double)pv;
+        const double qv = di.d * p64 - rb_sum(tabs.ftab, code, o);
+        st(L.q + d.off_c, rb_index(d, k, y, z), (float)qv);
+        acc[0].add(p64, qv);
+    };
+    // prologue: p(z0-2 .. z0+1), then Y(z0-1) and Y(z0), then p(z0+2) over p(z0-2)
+    {
+        float h[PR];
+        int cb[YR];
+        for (int zc = z0 - 2; zc <= z0 + 1; ++zc) {
+            p_load(zc, h);
+            p_store(zc, h);
+        }
+        __syncthreads();
+        yc_load(z0 - 1, cb);
+        y_plane(z0 - 1, cb);
+        yc_load(z0, cb);
+        y_plane(z0, cb);
+        p_load(z0 + 2, h);
+        __syncthreads();
+        p_store(z0 + 2, h);
+        __syncthreads();
+    }
+    float ha[PR];
+    int cba[YR], cbb[YR];
+    int rca, rcb;
+    yc_load(z0 + 1, cba);
+    rca = rc_load(z0);
+    // unrolled by 2 over alternating register sets, no early exit (a step past the last plane
+    // computes a Y plane and stores a p plane nobody reads, and forms no q)
+    for (int z = z0; z < z1; z += 2) {
+        p_load(z + 3, ha);
+        yc_load(z + 2, cbb);
+        rcb = rc_load(z + 1);
+        y_plane(z + 1, cba);
+        const float pva = p_own(z);
+        p_store(z + 3, ha);
+        __syncthreads();
+        q_plane(z, rca, pva);
+        p_load(z + 4, ha);
+        yc_load(z + 3, cba);
+        rca = rc_load(z + 2);
+        y_plane(z + 2, cbb);
+        const float pvb = p_own(z + 1);
+        p_store(z + 4, ha);
+        __syncthreads();
+        q_plane(z + 1, rcb, pvb);
+    }
+    block_reduce_store<AccDD, 1, NT>(acc, L.part + (long long)tile * kPart);
+}
+
 // The passes outside the Krylov loop, one thread per slot of a row, marching tz planes:
 //   CODE:    per-colour stencil codes from the full-grid codes (set-up)
 //   UPD:     x64 += x (kept unless discarded), x = 0                      [update / fresh start]
@@ -2105,6 +2325,36 @@ void rb_launch_it(const RbLaunch& L, hipStream_t s) {
 
 }  // namespace
 
+namespace {
+// Streaming kernels of known traffic, the measured ceiling the stencil passes are compared
+// with: c = a + s b (two reads, one write), b = a (one read, one write), a = s (one write);
+// 16-byte accesses, one per thread per sweep, the grid striding when it is capped.
+__global__ __launch_bounds__(256) void k_stream_triad(const float4* __restrict__ a, const float4* __restrict__ b,
+                                                      float4* __restrict__ c, size_t n4, float sc) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n4; i += (size_t)gridDim.x * 256) {
+        const float4 x = a[i], y = b[i];
+        c[i] = make_float4(x.x + sc * y.x, x.y + sc * y.y, x.z + sc * y.z, x.w + sc * y.w);
+    }
+}
+__global__ __launch_bounds__(256) void k_stream_copy(const float4* __restrict__ a, float4* __restrict__ b, size_t n4) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n4; i += (size_t)gridDim.x * 256) b[i] = a[i];
+}
+__global__ __launch_bounds__(256) void k_stream_fill(float4* __restrict__ a, size_t n4, float sc) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n4; i += (size_t)gridDim.x * 256)
+        a[i] = make_float4(sc, sc, sc, sc);
+}
+}  // namespace
+
+// kind 0 triad, 1 copy, 2 fill over buffers of n4 float4; blocks = 0: one float4 per thread.
+hipError_t stream_kernel(int kind, float4* a, float4* b, float4* c, size_t n4, unsigned blocks, hipStream_t s) {
+    const size_t need = (n4 + 255) / 256;
+    const unsigned g = blocks ? blocks : (unsigned)std::min<size_t>(need, 0x7FFFFFFFu);
+    if (kind == 0) hipLaunchKernelGGL(k_stream_triad, dim3(g), dim3(256), 0, s, a, b, c, n4, 0.5f);
+    else if (kind == 1) hipLaunchKernelGGL(k_stream_copy, dim3(g), dim3(256), 0, s, a, b, n4);
+    else hipLaunchKernelGGL(k_stream_fill, dim3(g), dim3(256), 0, s, a, n4, 0.5f);
+    return hipGetLastError();
+}
+
 hipError_t rb_setup(const RbLaunch& L, hipStream_t s) {
     if (L.nvol <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_rb_tables, dim3(L.nvol), dim3(256), 0, s, L.vol, L.coef, L.tab, L.nvol);
@@ -2113,7 +2363,9 @@ hipError_t rb_setup(const RbLaunch& L, hipStream_t s) {
 }
 
 static void rb_launch_q(const RbLaunch& L, hipStream_t s) {
-    if (L.staged)
+    if (L.staged == 2)
+        hipLaunchKernelGGL(k_rb_yq, dim3(L.ntiles), dim3(kRbTX * kRbTY), 0, s, L);
+    else if (L.staged)
         hipLaunchKernelGGL(k_rb_q, dim3(L.ntiles), dim3(kRbTX * kRbTY), 0, s, L);
     else
         rb_launch_it<RB_Q>(L, s);
@@ -2121,7 +2373,7 @@ static void rb_launch_q(const RbLaunch& L, hipStream_t s) {
 
 hipError_t rb_iter(const RbLaunch& L, hipStream_t s) {
     rb_launch_it<RB_P>(L, s);
-    rb_launch_it<RB_Y>(L, s);
+    if (L.staged != 2) rb_launch_it<RB_Y>(L, s);   // the fused pass computes Y itself
     rb_launch_q(L, s);
     hipLaunchKernelGGL((k_rb_fin<F_CG1>), dim3(L.nvol), dim3(kFinThreads), 0, s, L);
     rb_launch_it<RB_U>(L, s);
@@ -2140,7 +2392,9 @@ hipError_t rb_update(const RbLaunch& L, hipStream_t s) {
 hipError_t rb_pass_only(const RbLaunch& L, int k, hipStream_t s) {
     switch (k) {
         case 0: rb_launch_it<RB_P>(L, s); break;
-        case 1: rb_launch_it<RB_Y>(L, s); break;
+        case 1:
+            if (L.staged != 2) rb_launch_it<RB_Y>(L, s);
+            break;
         case 2: rb_launch_q(L, s); break;
         case 3: rb_launch_it<RB_U>(L, s); break;
         default: return hipErrorInvalidValue;

@@ -48,9 +48,10 @@ enum fast_dir { FAST_X_POS = 0, FAST_X_NEG = 1, FAST_Y_POS = 2, FAST_Y_NEG = 3, 
 /* DiffusionSolverType, VolumeMeasures.h:15-18 */
 enum fast_solver { FAST_BICGSTAB = 0, FAST_CG = 1 };
 /* Precision: FP64 and FP32 are the reference's `-p double|float` (TauModule.h:19,125-129);
- * MIXED is this library's form: BiCGSTAB with FP32 vectors and stencil, FP64 dot products
- * and scalars, the solution accumulated in FP64 and the residual periodically replaced by
- * the FP64 true residual (iterative refinement inside one Krylov process). */
+ * MIXED is this library's form: FP32 vectors (BiCGSTAB's directions p and v in FP64), FP64
+ * arithmetic, dot products and scalars, the solution accumulated in FP64 and the residual
+ * periodically replaced by the FP64 true residual (iterative refinement inside one Krylov
+ * process); CG runs on the red-black Schur complement unless mixed_full_grid is set. */
 enum fast_precision { FAST_FP64 = 0, FAST_FP32 = 1, FAST_MIXED = 2 };
 /* Solve status (Solver<T>::Status, fastlib/include/fastlib/solver/Solver.h:47-51, extended) */
 enum fast_status { FAST_SOLVED = 0, FAST_NO_CONVERGENCE = 1, FAST_DIVERGED = 2, FAST_FAILED = 3 };
@@ -103,7 +104,10 @@ typedef struct fast_ctx fast_ctx;
  * CUDA state (Diffusion.cu:8 __constant__ params, default stream).
  * Environment: FAST_TAU_GRAPHS=0 launches every polling round directly instead of replaying
  * a captured HIP graph (same kernels, same results); FAST_TAU_TUNING names a tile-shape table;
- * FAST_TAU_MIXED_RESTART sets the mixed form's shadow-residual reset threshold;
+ * FAST_TAU_MIXED_RESTART sets the mixed BiCGSTAB's shadow-residual reset threshold (a cosine,
+ * default 0: only Eigen's rule resets it); FAST_TAU_MIXED_BLOWUP g drops a mixed BiCGSTAB
+ * correction whose residual grew g times since the last reliable update (default: only a
+ * non-finite residual does);
  * FAST_TAU_GROUP_MB (default 320) is the red-black state size of the consecutive sub-batches a
  * batch of large volumes runs in (0 = always one launch; results are the same either way);
  * FAST_TAU_UPDATE_EVERY (default 4) is the number of mixed-form iterations between launches of
@@ -238,7 +242,7 @@ typedef struct fast_server_config {
     size_t hbm_budget;      /* bytes of the Krylov-state arena (0 = half the context budget, at
                              most 16 GiB); a request takes 46 B per element of its padded
                              per-colour layout, (ceil(nx/2)+2)(ny+2)(nz+2) elements (about
-                             23 B per voxel), with CG; 40 B per voxel with BiCGSTAB */
+                             23 B per voxel), with CG; 56 B per voxel with BiCGSTAB */
     int max_active;         /* max concurrently solving requests (0 = 64) */
     int iters_per_round;    /* Krylov iterations between admission points (0 = 8) */
     int use_bicgstab;       /* 0: CG (default), 1: BiCGSTAB */
@@ -320,6 +324,12 @@ int fast_debug_time_passes(fast_volume* vol, int dir, int precision, int solver,
  * update, r.r, r.D^-1 r), each launched `iters` times back to back; ms[4] = mean period of a
  * whole iteration.  ms: 5 doubles. */
 int fast_debug_time_rb_passes(fast_volume* vol, int dir, int iters, double* ms);
+/* Measured HBM ceiling: streaming kernels over three device buffers of `bytes` each (16-byte
+ * accesses), each launched `iters` times back to back after one untimed launch (HIP events):
+ * ms[0] c = a + s b with one float4 per thread, ms[1] the same on 8192 striding blocks,
+ * ms[2] b = a, ms[3] a = s.  Bytes moved per launch: 3, 3, 2 and 1 times `bytes`.  No
+ * reference counterpart.  ms: 4 doubles. */
+int fast_debug_stream(fast_ctx* ctx, size_t bytes, int iters, double* ms);
 /* `fast alpha` kernels, each run `iters` times back to back on the context's stream (HIP
  * events) after one untimed run: ms[0] = whole-volume area (area pass + final sum), ms[1] = the six per-face areas
  * from the cached labels (face codes + area pass + final sums), ms[2] = the six per-face

@@ -74,29 +74,37 @@ def test_mixed_cg_batch_within_1e5_of_converged(ctx, batch96):
     assert errs.max() <= 1e-5
 
 
-def test_mixed_bicgstab_batch(ctx, batch96):
-    """Mixed BiCGSTAB on the same batch at 1e-7, beside FP64 BiCGSTAB at 1e-7.  BiCGSTAB's
-    residual test does not bound tau well on these high-contrast volumes even in FP64 (errors
-    up to ~1e-3 at 1e-7), and the mixed form, whose FP32-stored shadow residual loses
-    bi-orthogonality, does not converge on some of them within 20000 iterations; mixed CG is
-    the mixed solver to use.  What must hold: a volume that does not converge reports it
-    (status 1, tau 0, as the reference returns 0), and converged volumes are no worse than
-    10x the FP64 BiCGSTAB's worst error at the same tolerance.  The numbers are printed."""
+@pytest.mark.parametrize("tol", [1e-7, 1e-8])
+def test_mixed_bicgstab_batch_converges_on_every_volume(ctx, batch96, tol):
+    """The batched workload's named solver: mixed BiCGSTAB (FP32 residual, shadow residual, s
+    and correction; FP64 directions p and v, arithmetic and reliable updates) converges on all
+    96 volumes.  BiCGSTAB's residual test bounds tau loosely on these high-contrast volumes
+    even in FP64 (FP64 BiCGSTAB: up to 6.2e-4 off at 1e-7, 2.5e-4 at 1e-8), so the bar here is
+    the FP64 BiCGSTAB's own worst error at the same tolerance, times 10."""
     items, vols, dirs, conv = batch96
-    print("FP64 / mixed BiCGSTAB batches: solving", flush=True)
-    ref = T.solve_batch(ctx, vols, [T.make_params(d, BICG, "double", 1e-7, 20000) for d in dirs])
-    out = T.solve_batch(ctx, vols, [T.make_params(d, BICG, "mixed", 1e-7, 20000) for d in dirs])
+    print(f"FP64 / mixed BiCGSTAB batches at {tol}: solving", flush=True)
+    ref = T.solve_batch(ctx, vols, [T.make_params(d, BICG, "double", tol, 20000) for d in dirs])
+    out = T.solve_batch(ctx, vols, [T.make_params(d, BICG, "mixed", tol, 20000) for d in dirs])
     e_ref, e_mix = _errs(ref, conv), _errs(out, conv)
-    ok = np.array([r.status == 0 for r in out])
-    print(f"BiCGSTAB 1e-7: fp64 converged {sum(r.status == 0 for r in ref)}/{len(ref)}, max err {e_ref.max():.2e}, "
-          f"median {np.median(e_ref):.2e}; mixed converged {ok.sum()}/{len(out)}, max err of converged "
-          f"{e_mix[ok].max():.2e}, median {np.median(e_mix[ok]):.2e}; not converged: "
-          f"{[(int(i), items[i].mask.shape, round(items[i].r, 2)) for i in np.where(~ok)[0]]}")
+    print(f"BiCGSTAB {tol}: fp64 max err {e_ref.max():.2e}, median {np.median(e_ref):.2e}, batch {ref[0].solve_ms:.0f} ms; "
+          f"mixed converged {sum(r.status == 0 for r in out)}/96, max err {e_mix.max():.2e}, median "
+          f"{np.median(e_mix):.2e}, iterations {min(r.iterations for r in out)}..{max(r.iterations for r in out)}, "
+          f"batch {out[0].solve_ms:.0f} ms")
     assert all(r.status == 0 for r in ref)
     for i, r in enumerate(out):
-        if r.status != 0:
-            assert r.status == 1 and r.tau == 0.0, (i, r.status, r.tau)
-    assert e_mix[ok].max() <= 10 * e_ref.max()
+        assert r.status == 0 and r.true_rel_residual <= tol, (i, items[i].mask.shape, items[i].r, r.iterations)
+    assert e_mix.max() <= 10 * e_ref.max()
+
+
+def test_mixed_bicgstab_batch_meets_the_tau_bar_at_1e10(ctx, batch96):
+    """At a tolerance of 1e-10 every mixed BiCGSTAB tau is within 1e-5 of the oracle's
+    converged tau (measured worst 1.3e-6)."""
+    items, vols, dirs, conv = batch96
+    out = T.solve_batch(ctx, vols, [T.make_params(d, BICG, "mixed", 1e-10, 20000) for d in dirs])
+    errs = _errs(out, conv)
+    print(f"mixed BiCGSTAB 1e-10: max err {errs.max():.2e}, batch {out[0].solve_ms:.0f} ms")
+    assert all(r.status == 0 for r in out)
+    assert errs.max() <= 1e-5
 
 
 def test_fp64_batch_equals_solo(ctx, batch96):

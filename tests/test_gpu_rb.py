@@ -118,26 +118,47 @@ def test_reduced_batch_equals_solo_uneven(ctx):
         v.close()
 
 
-def test_staged_q_pass_bit_identical(ctx, monkeypatch):
-    """The Q pass with the black Y planes staged in LDS and the one that reads the six Y
-    neighbours from memory compute the same values in the same order: tau, iterations and the
-    concentration field are identical, on a cube and on odd shapes whose tiles are ragged in
-    every axis."""
+def test_y_and_q_forms_bit_identical(ctx, monkeypatch):
+    """The three forms of the Y and Q passes compute the same values in the same order: Y and
+    Q separate with Q reading the six Y neighbours from memory (0), Q staging the Y planes in
+    LDS (1), and Y fused into Q (2, the default).  tau, iterations and the concentration field
+    are identical, on cubes and on odd shapes whose tiles are ragged in every axis, with tiles
+    of a few planes and of many."""
     cases = [F.generate_spheres(64, 6.0, 0.60, 0), rand_mask((19, 23, 37), 0.45, 1), rand_mask((11, 17, 131), 0.4, 5),
-             rand_mask((70, 5, 1), 0.3, 3)]
+             rand_mask((70, 5, 1), 0.3, 3), rand_mask((33, 9, 66), 0.5, 7), rand_mask((1, 1, 40), 0.2, 8)]
     for m in cases:
         vol = ctx.volume(m)
         for d in (0, 3, 4):
             p = T.make_params(d, CG, "mixed", 1e-9, 10000)
-            monkeypatch.setenv("FAST_TAU_RB_STAGED", "1")
-            a, ca = vol.solve(p, conc=True)
-            monkeypatch.setenv("FAST_TAU_RB_STAGED", "0")
-            b, cb = vol.solve(p, conc=True)
+            out = []
+            for mode in ("0", "1", "2"):
+                monkeypatch.setenv("FAST_TAU_RB_STAGED", mode)
+                out.append(vol.solve(p, conc=True))
             monkeypatch.delenv("FAST_TAU_RB_STAGED")
-            assert a.status == 0 and b.status == 0
-            assert a.tau == b.tau and a.iterations == b.iterations, (m.shape, d, a.tau, b.tau)
-            assert np.array_equal(ca, cb), (m.shape, d)
+            (a, ca) = out[0]
+            assert a.status == 0
+            for b, cb in out[1:]:
+                assert b.status == 0
+                assert a.tau == b.tau and a.iterations == b.iterations, (m.shape, d, a.tau, b.tau)
+                assert np.array_equal(ca, cb), (m.shape, d)
         vol.close()
+
+
+def test_fused_form_bit_identical_with_forced_tile_depths(ctx, monkeypatch):
+    """Tile depths of 1 to 7 planes (FAST_TAU_RB_TZ) put the fused pass's prologue and the
+    unrolled march's spare step at every offset from the volume's last plane."""
+    m = rand_mask((13, 21, 70), 0.45, 9)
+    vol = ctx.volume(m)
+    p = T.make_params(2, CG, "mixed", 1e-9, 10000)
+    monkeypatch.setenv("FAST_TAU_RB_STAGED", "0")
+    ref, cref = vol.solve(p, conc=True)
+    monkeypatch.delenv("FAST_TAU_RB_STAGED")
+    for tz in range(1, 8):
+        monkeypatch.setenv("FAST_TAU_RB_TZ", str(tz))
+        got, c = vol.solve(p, conc=True)
+        assert got.tau == ref.tau and got.iterations == ref.iterations and np.array_equal(c, cref), tz
+    monkeypatch.delenv("FAST_TAU_RB_TZ")
+    vol.close()
 
 
 def test_large_volume_batch_runs_in_cache_sized_groups(monkeypatch):
