@@ -205,6 +205,23 @@ def serving(ctx, T, W, n_req: int, loads=(0.5, 0.8, 1.1), per_edge: int = 16, ra
     return out
 
 
+def hbm_ceiling(ctx, lib, gib: float = 2.0, iters: int = 10) -> dict:
+    """The measured HBM ceiling: hand-written streaming kernels (16-byte accesses) over three
+    buffers of `gib` GiB each, far larger than the 256 MB Infinity Cache, timed with HIP
+    events on the solve stream: triad c = a + s b (2 reads, 1 write), the same on a striding
+    grid of 8192 blocks, copy b = a, fill a = s."""
+    ms = (C.c_double * 4)()
+    nbytes = int(gib * 2 ** 30) // 16 * 16
+    rc = lib().fast_debug_stream(ctx.handle, nbytes, iters, ms)
+    if rc != 0:
+        return {"error": f"fast_debug_stream returned {rc}"}
+    moved = {"triad": 3 * nbytes, "triad_8192_blocks": 3 * nbytes, "copy": 2 * nbytes, "fill": nbytes}
+    out = {"buffer_bytes": nbytes, "timing": f"HIP events around {iters} back-to-back launches"}
+    for i, (k, b) in enumerate(moved.items()):
+        out[k] = {"ms": ms[i], "GBs": b / (ms[i] * 1e-3) / 1e9, "frac_of_peak": b / (ms[i] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    return out
+
+
 def measured_traffic(kernel: str, applies: bool):
     """HBM bytes per launch of `kernel` at 256^3, from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE counters (copy-calibrated); None when not measured."""
@@ -222,10 +239,19 @@ def profile_program(kind: str, n: int) -> int:
     from fast_amd import tau as T
     from fast_amd import workloads as W
     from fast_amd._lib import lib
-    mask = W.ellipsoids_256(0) if n == 256 else T.generate_ellipsoids(n, 12.0, 0.35, 0.65, 0)
+    mask = W.ellipsoids_256(0) if n == 256 else (W.needles_512(0) if n == 512 else
+                                                  T.generate_ellipsoids(n, 12.0, 0.35, 0.65, 0))
     ctx = T.Context(0)
     vol = ctx.volume(mask)
-    if kind == "solves":
+    if kind == "rb":
+        rb = (C.c_double * 5)()
+        lib().fast_debug_time_rb_passes(vol.handle, 0, 20, rb)
+        print(f"mixed cg red-black: P, Y, Q, U ms {list(rb)[:4]} iteration ms {rb[4]}", flush=True)
+        # calibration kernels of known traffic: triad (read 2 x 1 GiB, write 1 GiB), copy, fill
+        st = (C.c_double * 4)()
+        lib().fast_debug_stream(ctx.handle, 2 ** 30, 3, st)
+        print(f"stream 1 GiB buffers: triad, triad (8192 blocks), copy, fill ms {list(st)}", flush=True)
+    elif kind == "solves":
         for name, solver, prec, full in (("fp64 bicgstab", BICG, "double", False), ("fp64 cg", CG, "double", False),
                                          ("mixed cg red-black", CG, "mixed", False),
                                          ("mixed cg full grid", CG, "mixed", True)):
@@ -271,11 +297,12 @@ def main() -> int:
     ap.add_argument("--serve-pool", type=int, default=8, help="volumes per edge length in the request pool")
     ap.add_argument("--no-dram-check", dest="dram_check", action="store_false",
                     help="skip the red-black pass timings on the 512^3 needle volume")
-    ap.add_argument("--profile", choices=["solves", "passes"],
+    ap.add_argument("--profile", choices=["solves", "passes", "rb"],
                     help="short program for rocprofv3 instead of the benchmark: 'solves' runs the X_POS solves "
                          "(FP64 BiCGSTAB, FP64 CG, mixed CG red-black and full grid) with direct launches; 'passes' "
                          "times 20 launches of each solver's passes and runs a 256 MiB device copy for counter "
-                         "calibration")
+                         "calibration; 'rb' times 20 launches of each red-black pass and 3 of each streaming kernel over 1 GiB "
+                         "buffers (triad, copy, fill: the counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -410,6 +437,8 @@ def main() -> int:
                                 "achieved_GBs": b / (rb[i] * 1e-3) / 1e9, "frac": b / (rb[i] * 1e-3) / 8e12}
                             for i, (k, b) in b5.items()},
                 "iteration_ms": rb[4], "iteration_GBs": sum(b for _, b in b5.values()) / (rb[4] * 1e-3) / 1e9}
+        log("[bench] streaming kernels over 3 x 2 GiB (measured HBM ceiling)")
+        dram["hbm_ceiling"] = hbm_ceiling(ctx, lib)
 
     cpu = None
     if not args.no_cpu:

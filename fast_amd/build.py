@@ -20,7 +20,7 @@ OBJ = HERE / "_build"
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 ARCH = os.environ.get("FAST_TAU_ARCH", "gfx950")
 
-HIP_SOURCES = ["tau_kernels.hip", "ccl_kernels.hip", "alpha_kernels.hip"]
+HIP_SOURCES = ["tau_kernels.hip", "rb_kernels.hip", "ccl_kernels.hip", "alpha_kernels.hip"]
 CXX_SOURCES = ["tau_api.cpp", "ccl_api.cpp", "alpha_api.cpp", "server.cpp", "volume_io.cpp"]
 
 HIPCC_FLAGS = [

@@ -107,6 +107,7 @@ struct fast_server {
     fast_server_config cfg{};
     int solver = FAST_CG;
     bool rb = true;                           // CG on the red-black Schur complement
+    fth::RbShape rbs{4, 0, 2};                // red-black rows per tile and prefetch (tz per request)
     std::thread worker;
     std::mutex mu;
     std::condition_variable cv_in, cv_out;
@@ -233,7 +234,7 @@ int rebuild_rb(fast_server* s, const std::vector<Req*>& adm) {
         d.primary = r->prm.dir / 2;
         d.sgn = (r->prm.dir % 2 == 0) ? 1 : -1;
         d.tiles_x = (d.hx + kRbTX - 1) / kRbTX;
-        d.tiles_y = (r->ny + kRbTY - 1) / kRbTY;
+        d.tiles_y = (r->ny + s->rbs.ty - 1) / s->rbs.ty;
         d.tz = r->tz;
         d.tiles_z = (r->nz + d.tz - 1) / d.tz;
         d.ntiles = d.tiles_x * d.tiles_y * d.tiles_z;
@@ -270,6 +271,8 @@ int rebuild_rb(fast_server* s, const std::vector<Req*>& adm) {
     L.ntiles = tiles;
     L.nvol = s->max_active;
     L.staged = rb_mode();
+    L.ty = s->rbs.ty;
+    L.ahead = s->rbs.ahead;
     L.vol = s->d_rb;
     L.tile_vol = s->d_tile;
     L.coef = s->d_coef;
@@ -623,6 +626,9 @@ int fast_server_create(fast_ctx* ctx, const fast_server_config* cfg, fast_server
     if (s->cfg.iters_per_round <= 0) s->cfg.iters_per_round = 8;
     s->solver = s->cfg.use_bicgstab ? FAST_BICGSTAB : FAST_CG;
     s->rb = (s->solver == FAST_CG);
+    // one launch set serves every live request: the rows per tile and prefetch distance of the
+    // 256^3 size class (the middle of the request sizes), tz per request
+    s->rbs = rb_shape(ctx, 256, 256, 256);
     s->max_active = s->cfg.max_active;
     const size_t budget = s->cfg.hbm_budget ? s->cfg.hbm_budget : std::min(ctx->budget / 2, (size_t)16 << 30);
     // bytes per arena element: red-black, kRbBytes per padded per-colour element; full grid,
@@ -689,7 +695,7 @@ int fast_server_submit(fast_server* s, const uint8_t* mask, int nx, int ny, int 
     if (s->rb) {   // padded per-colour elements
         const int hx = (nx + 1) / 2;
         r->elems = align_up((size_t)(hx + 2) * (size_t)(ny + 2) * (size_t)(nz + 2), kAlign);
-        r->tz = rb_tz(hx, ny, nz);
+        r->tz = rb_shape(s->ctx, nx, ny, nz).tz;
     } else {
         r->elems = align_up(n, kAlign);
         r->tz = choose_shape(nx, ny, nz, FAST_MIXED, s->ctx).tz;

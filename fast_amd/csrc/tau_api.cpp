@@ -204,18 +204,52 @@ void load_tuning(fast_ctx* ctx) {
     FILE* f = path.empty() ? nullptr : std::fopen(path.c_str(), "r");
     if (!f) return;
     char key[64];
-    int tx, ty, tz;
-    while (std::fscanf(f, "%63s %d %d %d", key, &tx, &ty, &tz) == 4) {
-        if (shape_compiled(tx, ty) && tz >= 1 && tz <= 256) ctx->tuned[key] = Shape{tx, ty, tz};
+    int a, b, c;   // full grid: tx ty tz; red-black ("rb_" keys): ty tz ahead
+    while (std::fscanf(f, "%63s %d %d %d", key, &a, &b, &c) == 4) {
+        if (std::strncmp(key, "rb_", 3) == 0) {
+            if (rb_ty_compiled(a) && b >= 1 && b <= 256 && (c == 2 || c == 3)) ctx->rb_tuned[key] = RbShape{a, b, c};
+        } else if (shape_compiled(a, b) && c >= 1 && c <= 256) {
+            ctx->tuned[key] = Shape{a, b, c};
+        }
     }
     std::fclose(f);
 }
 
+static int size_class(int e) { return e <= 16 ? 16 : e <= 32 ? 32 : e <= 64 ? 64 : e <= 128 ? 128 : e <= 256 ? 256 : 512; }
+
 std::string shape_key(int nx, int ny, int nz, int precision) {
-    auto cls = [](int e) { return e <= 16 ? 16 : e <= 32 ? 32 : e <= 64 ? 64 : e <= 128 ? 128 : e <= 256 ? 256 : 512; };
     char b[64];
-    std::snprintf(b, sizeof(b), "%d_%d_%d_%d", cls(nx), cls(ny), cls(nz), precision);
+    std::snprintf(b, sizeof(b), "%d_%d_%d_%d", size_class(nx), size_class(ny), size_class(nz), precision);
     return b;
+}
+
+std::string rb_key(int nx, int ny, int nz) {
+    char b[64];
+    std::snprintf(b, sizeof(b), "rb_%d_%d_%d", size_class(nx), size_class(ny), size_class(nz));
+    return b;
+}
+
+bool rb_ty_compiled(int ty) { return ty == 2 || ty == 4 || ty == 8; }
+
+int rb_tz(int hx, int ny, int nz, int ty) {
+    const long long txy = (long long)((hx + kRbTX - 1) / kRbTX) * ((ny + ty - 1) / ty);
+    return (int)std::max<long long>(4, std::min<long long>(32, (long long)nz * txy / 2048));
+}
+
+RbShape rb_shape(const fast_ctx* ctx, int nx, int ny, int nz) {
+    RbShape r{4, 0, 2};
+    if (ctx->rb_forced.ty) {
+        r = ctx->rb_forced;
+    } else {
+        auto it = ctx->rb_tuned.find(rb_key(nx, ny, nz));
+        if (it != ctx->rb_tuned.end()) r = it->second;
+    }
+    if (r.tz <= 0) r.tz = rb_tz((nx + 1) / 2, ny, nz, r.ty);
+    if (const char* e = std::getenv("FAST_TAU_RB_TZ")) {
+        const int v = std::atoi(e);
+        if (v > 0) r.tz = v;
+    }
+    return r;
 }
 
 }  // namespace fth
@@ -317,6 +351,17 @@ int fast_ctx_set_tile_shape(fast_ctx* ctx, int tx, int ty, int tz) {
     }
     if (!shape_compiled(tx, ty) || tz < 1 || tz > 256) return FAST_E_BADARG;
     ctx->forced = Shape{tx, ty, tz};
+    return FAST_OK;
+}
+
+int fast_ctx_set_rb_shape(fast_ctx* ctx, int ty, int tz, int ahead) {
+    if (!ctx) return FAST_E_BADARG;
+    if (ty == 0) {
+        ctx->rb_forced = RbShape{0, 0, 0};
+        return FAST_OK;
+    }
+    if (!rb_ty_compiled(ty) || tz < 0 || tz > 256 || (ahead != 2 && ahead != 3)) return FAST_E_BADARG;
+    ctx->rb_forced = RbShape{ty, tz, ahead};
     return FAST_OK;
 }
 
@@ -853,16 +898,6 @@ int rb_mode() {
     return e ? std::min(2, std::max(0, std::atoi(e))) : 2;
 }
 
-// Planes per RB tile: enough tiles to fill the chip (about 2048), 4..32 planes each.
-int rb_tz(int hx, int ny, int nz) {
-    if (const char* e = std::getenv("FAST_TAU_RB_TZ")) {
-        const int v = std::atoi(e);
-        if (v > 0) return v;
-    }
-    const long long txy = (long long)((hx + kRbTX - 1) / kRbTX) * ((ny + kRbTY - 1) / kRbTY);
-    return (int)std::max<long long>(4, std::min<long long>(32, (long long)nz * txy / 2048));
-}
-
 static int prepare_rb(RbBatch& B) {
     fast_ctx* ctx = B.ctx;
     const int nv = (int)B.jobs.size();
@@ -870,6 +905,12 @@ static int prepare_rb(RbBatch& B) {
     B.hvol.resize(nv);
     B.hcoef.resize(nv);
     B.hst.assign(nv, VolState<double>{});
+    // rows per tile and prefetch distance: those of the largest volume (one launch set serves
+    // the batch); planes per tile: each volume's own
+    const fast_volume* big = B.jobs[0]->vol;
+    for (const Job* j : B.jobs)
+        if ((size_t)j->vol->nx * j->vol->ny * j->vol->nz > (size_t)big->nx * big->ny * big->nz) big = j->vol;
+    const RbShape bs = rb_shape(ctx, big->nx, big->ny, big->nz);
     int tiles = 0, planes = 0;
     size_t total = 0, total_c = 0;
     for (int i = 0; i < nv; ++i) {
@@ -891,8 +932,8 @@ static int prepare_rb(RbBatch& B) {
         r.primary = j.prm.dir / 2;
         r.sgn = (j.prm.dir % 2 == 0) ? 1 : -1;
         r.tiles_x = (r.hx + kRbTX - 1) / kRbTX;
-        r.tiles_y = (v->ny + kRbTY - 1) / kRbTY;
-        r.tz = rb_tz(r.hx, v->ny, v->nz);
+        r.tiles_y = (v->ny + bs.ty - 1) / bs.ty;
+        r.tz = rb_shape(ctx, v->nx, v->ny, v->nz).tz;
         r.tiles_z = (v->nz + r.tz - 1) / r.tz;
         r.tile_base = tiles;
         r.ntiles = r.tiles_x * r.tiles_y * r.tiles_z;
@@ -994,6 +1035,8 @@ static int prepare_rb(RbBatch& B) {
     L.ntiles = tiles;
     L.nvol = nv;
     L.staged = rb_mode();
+    L.ty = bs.ty;
+    L.ahead = bs.ahead;
     L.vol = (const RbDesc*)(m + o_rb);
     L.tile_vol = (const int*)(m + o_tile);
     L.coef = (const Coef<double>*)(m + o_coef);

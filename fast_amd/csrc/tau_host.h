@@ -25,6 +25,12 @@ struct Shape {
     int tx, ty, tz;
 };
 
+// Tile shape of the red-black passes: ty rows of kRbTX slots per tile, tz planes per tile,
+// and the fused Y+Q pass's prefetch distance in planes.
+struct RbShape {
+    int ty, tz, ahead;
+};
+
 struct PinBuf {
     void* ptr = nullptr;
     size_t size = 0;
@@ -47,6 +53,8 @@ struct fast_ctx {
     unsigned long long* zero_count = nullptr;
     double* alpha_scratch = nullptr;           // this device's copy of the area kernels' static scratch
     std::map<std::string, fth::Shape> tuned;   // autotuned tile shapes by size class
+    std::map<std::string, fth::RbShape> rb_tuned;   // autotuned red-black tile shapes by size class
+    fth::RbShape rb_forced{0, 0, 0};           // red-black shape forced by fast_ctx_set_rb_shape
     fth::Shape forced{0, 0, 0};                // tile shape forced by fast_ctx_set_tile_shape
     int chunk = 16;                            // Krylov iterations per polling round
     bool use_graphs = true;                    // replay polling rounds from a captured HIP graph
@@ -101,8 +109,14 @@ std::string shape_key(int nx, int ny, int nz, int precision);
 Shape choose_shape(int nx, int ny, int nz, int precision, const fast_ctx* ctx);
 void load_tuning(fast_ctx* ctx);
 int solve_jobs(fast_ctx* ctx, std::vector<Job>& jobs);
-// Planes per tile of the red-black passes for a volume with hx slots per row.
-int rb_tz(int hx, int ny, int nz);
+// Red-black tile shape of a volume: forced (fast_ctx_set_rb_shape), else the tuned entry of
+// its size class (key "rb_<x>_<y>_<z>"), else 4 rows, the planes rb_tz gives and 2 planes of
+// prefetch.  $FAST_TAU_RB_TZ overrides the planes per tile.
+RbShape rb_shape(const fast_ctx* ctx, int nx, int ny, int nz);
+// Planes per tile for a volume with hx slots per row: enough tiles to fill the chip.
+int rb_tz(int hx, int ny, int nz, int ty);
+std::string rb_key(int nx, int ny, int nz);
+bool rb_ty_compiled(int ty);
 // Form of the red-black Y and Q passes (RbLaunch::staged): FAST_TAU_RB_STAGED = 0 (separate,
 // Q reads six Y values), 1 (separate, Q stages the Y planes in LDS), default 2 (fused).
 int rb_mode();

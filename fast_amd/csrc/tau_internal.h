@@ -184,6 +184,8 @@ struct RbDesc {
 
 struct RbLaunch {
     int ntiles, nvol;
+    int ty;           // rows per tile (2, 4 or 8) of every red-black pass of this launch set
+    int ahead;        // fused Y+Q pass: planes its loads run ahead (2 or 3)
     int staged;       // 2: Y and Q fused (no Y vector); 1: Q pass with the Y planes staged in LDS; 0: six
                       // Y loads per voxel
     const RbDesc* vol;
@@ -203,7 +205,7 @@ struct RbLaunch {
     double* xfull;    // full grid: FP64 solution (for the flux integral)
 };
 
-constexpr int kRbTX = 64, kRbTY = 4;   // threads per block along k and y
+constexpr int kRbTX = 64;   // threads per block along k (one wave per row of a tile)
 
 // Diagonal tables and per-colour codes (after the vectors and codes were zero-filled).
 hipError_t rb_setup(const RbLaunch&, hipStream_t);

@@ -127,6 +127,12 @@ size_t fast_ctx_device_bytes(const fast_ctx* ctx);
  * tx = 0 restores the automatic choice (the autotuned table, else a size rule).  Used by
  * the autotuner (fast_amd/autotune.py).  No reference counterpart. */
 int fast_ctx_set_tile_shape(fast_ctx* ctx, int tx, int ty, int tz);
+/* Force the tile shape of subsequent red-black (mixed CG) solves: ty rows of 64 slots per
+ * tile (2, 4 or 8), tz planes per tile (1..256; 0 = the size rule) and the fused Y+Q pass's
+ * prefetch distance in planes (2 or 3).  ty = 0 restores the automatic choice (the autotuned
+ * "rb_" table entries, else 4 rows, the size rule and 2).  Used by the autotuner
+ * (fast_amd/autotune.py).  No reference counterpart. */
+int fast_ctx_set_rb_shape(fast_ctx* ctx, int ty, int tz, int ahead);
 /* Polling-round length in Krylov iterations (captured as one HIP graph); 0 = default 16. */
 int fast_ctx_set_round(fast_ctx* ctx, int iterations);
 

@@ -60,10 +60,11 @@ def _errs(out, conv):
 
 def test_mixed_cg_batch_within_1e5_of_converged(ctx, batch96):
     """The plate-like volumes (large r) are ill-conditioned: a relative residual of 1e-7 leaves
-    tau up to ~1e-4 off there, so the batch is solved to 1e-8."""
+    tau up to ~1e-4 off there, and 1e-8 up to ~5e-5 (where in that range a solve stops depends
+    on its rounding), so the batch is solved to 1e-9."""
     items, vols, dirs, conv = batch96
     print("mixed CG batch: solving", flush=True)
-    out = T.solve_batch(ctx, vols, [T.make_params(d, CG, "mixed", 1e-8, 60000) for d in dirs])
+    out = T.solve_batch(ctx, vols, [T.make_params(d, CG, "mixed", 1e-9, 60000) for d in dirs])
     for i, r in enumerate(out):
         assert r.status == 0, (i, items[i].mask.shape, items[i].r, r.iterations)
     errs = _errs(out, conv)

@@ -144,6 +144,25 @@ def test_y_and_q_forms_bit_identical(ctx, monkeypatch):
         vol.close()
 
 
+def test_tile_shapes_bit_identical(ctx):
+    """Rows per tile (2, 4, 8), planes per tile and the fused pass's prefetch distance change
+    only the launch geometry: every dot product is rounded once from a double-double sum, so
+    tau, iterations and the field are identical, on a cube and on shapes ragged in every axis."""
+    cases = [F.generate_spheres(48, 5.0, 0.55, 2), rand_mask((21, 13, 29), 0.45, 11), rand_mask((66, 9, 5), 0.35, 12)]
+    for m in cases:
+        vol = ctx.volume(m)
+        p = T.make_params(4, CG, "mixed", 1e-9, 10000)
+        ref, cref = vol.solve(p, conc=True)
+        assert ref.status == 0
+        for ty, tz, ahead in ((2, 0, 3), (8, 0, 3), (4, 0, 2), (2, 3, 2), (8, 5, 3), (4, 32, 3)):
+            ctx.set_rb_shape(ty, tz, ahead)
+            got, c = vol.solve(p, conc=True)
+            ctx.set_rb_shape(0)
+            assert got.tau == ref.tau and got.iterations == ref.iterations, (m.shape, ty, tz, ahead)
+            assert np.array_equal(c, cref), (m.shape, ty, tz, ahead)
+        vol.close()
+
+
 def test_fused_form_bit_identical_with_forced_tile_depths(ctx, monkeypatch):
     """Tile depths of 1 to 7 planes (FAST_TAU_RB_TZ) put the fused pass's prologue and the
     unrolled march's spare step at every offset from the volume's last plane."""
