@@ -469,6 +469,8 @@ def main() -> int:
             "workload": f"{n}^3 ellipsoids a=12 r=0.35 solid 0.65 seed 0; tol 1e-6; x,y,z solved as one "
                         "batched mixed-precision CG solve (red-black Schur complement) per step",
             "volume": [n, n, n], "solves_per_step": len(DIRS), "parallelism": "single GPU",
+            "red_black_tile": dict(zip(("rows", "planes", "prefetch_planes"), ctx.rb_shape((n, n, n)))),
+            "red_black_tile_source": "autotuned table (fast_amd/tuning_gfx950.txt, rb_ keys) or the size rule",
         },
         "mixed_batched_ms_per_solve": ms_per_step / len(DIRS),
         "mixed_full_grid_batched_solves_per_s": full_rate,

@@ -105,6 +105,7 @@ PROTOTYPES = [
     ("fast_ctx_device_bytes", C.c_size_t, [_vp]),
     ("fast_ctx_set_tile_shape", C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     ("fast_ctx_set_rb_shape", C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    ("fast_ctx_rb_shape", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
     ("fast_ctx_set_round", C.c_int, [_vp, C.c_int]),
     ("fast_volume_create", C.c_int, [_vp, _u8p, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
     ("fast_volume_create_device", C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),

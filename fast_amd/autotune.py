@@ -1,13 +1,21 @@
-"""Autotuner of the tile shape (block tx x ty, tz planes per tile) per volume size class.
+"""Autotuner of the tile shapes per volume size class.
 
-For every size class (per-axis edge class 32/64/128/256/512, precision) it times one
-Krylov iteration of the solver that precision mostly runs (FP64: BiCGSTAB, mixed: CG) with
-HIP events for every compiled block shape and a range of tz, and writes the fastest as
-lines "<key> <tx> <ty> <tz>" to tuning_gfx950.txt next to libfasttau.so, where the library
-picks them up (or wherever $FAST_TAU_TUNING points).  Tile shapes never change results:
-every dot product is rounded once from a double-double sum.
+Full grid (FP64 and FP32/mixed BiCGSTAB, mixed CG with mixed_full_grid): for every size
+class (per-axis edge class 32/64/128/256/512, precision) it times one Krylov iteration of
+the solver that precision mostly runs (FP64: BiCGSTAB, mixed: CG) with HIP events for every
+compiled block shape (tx x ty) and a range of tz, and writes the fastest as lines
+"<key> <tx> <ty> <tz>".
 
-  python -m fast_amd.autotune [--classes 32,64,128,256] [--out PATH]
+Red-black mixed CG (the default mixed CG): for every size class it times one red-black
+iteration (the P, fused Y+Q and U passes with both finalizes) for every rows-per-tile (2, 4,
+8), planes per tile and fused-pass prefetch distance (2, 3), and writes the fastest as lines
+"rb_<x>_<y>_<z> <ty> <tz> <ahead>".
+
+The lines go to tuning_gfx950.txt next to libfasttau.so, where the library picks them up
+(or wherever $FAST_TAU_TUNING points).  Tile shapes never change results: every dot product
+is rounded once from a double-double sum.
+
+  python -m fast_amd.autotune [--classes 32,64,128,256] [--rb-classes 32,...,512] [--out PATH]
 """
 from __future__ import annotations
 
@@ -47,6 +55,51 @@ def time_iteration(ctx, vol, precision: int, solver: int, iters: int) -> float:
     return ms[3]
 
 
+RB_TY = (2, 4, 8)
+RB_TZ = (4, 8, 16, 32, 64)
+RB_AHEAD = (2, 3)
+
+
+def rb_key(nx: int, ny: int, nz: int) -> str:
+    return f"rb_{cls(nx)}_{cls(ny)}_{cls(nz)}"
+
+
+def time_rb_iteration(ctx, vol, iters: int) -> float:
+    ms = (C.c_double * 5)()
+    rc = lib().fast_debug_time_rb_passes(vol.handle, 0, iters, ms)
+    if rc != 0:
+        raise RuntimeError(f"timing failed ({rc}): {ctx.last_error()}")
+    return ms[4]
+
+
+def tune_rb(classes, iters: int = 10, verbose: bool = True) -> dict:
+    """Red-black shapes: every (ty, tz, ahead) per size class; cubes of class 512 only."""
+    ctx = T.Context(0)
+    best = {}
+    for dims in itertools.product(classes, repeat=3):
+        if max(dims) >= 512 and min(dims) < 512:
+            continue
+        nx, ny, nz = dims
+        m = T.generate_ellipsoids((nx, ny, nz), 12.0, 0.35, 0.6, 1)
+        vol = ctx.volume(m)
+        results = []
+        for ty, tz, ahead in itertools.product(RB_TY, RB_TZ, RB_AHEAD):
+            if tz > nz or ty > 2 * ny:
+                continue
+            ctx.set_rb_shape(ty, tz, ahead)
+            results.append((time_rb_iteration(ctx, vol, iters), ty, tz, ahead))
+        ctx.set_rb_shape(0)
+        results.sort()
+        t, ty, tz, ahead = results[0]
+        best[rb_key(nx, ny, nz)] = (ty, tz, ahead, t)
+        if verbose:
+            print(f"{rb_key(nx, ny, nz)}: best ty={ty} tz={tz} ahead={ahead} {t * 1e3:.1f} us/iteration "
+                  f"(worst {results[-1][0] * 1e3:.1f})", flush=True)
+        vol.close()
+    ctx.close()
+    return best
+
+
 def tune(classes, out: Path, iters: int = 12, verbose: bool = True) -> dict:
     ctx = T.Context(0)
     best = {}
@@ -76,20 +129,35 @@ def tune(classes, out: Path, iters: int = 12, verbose: bool = True) -> dict:
                       f"(worst {results[-1][0] * 1e3:.1f})", flush=True)
         vol.close()
     ctx.close()
-    with open(out, "w") as f:
-        for k, (tx, ty, tz, t) in sorted(best.items()):
-            f.write(f"{k} {tx} {ty} {tz}\n")
     return best
+
+
+def write_table(out: Path, grid: dict, rb: dict) -> None:
+    """Merge new entries into the table at `out` (entries of other keys are kept)."""
+    lines = {}
+    if out.exists():
+        for line in out.read_text().splitlines():
+            parts = line.split()
+            if len(parts) == 4:
+                lines[parts[0]] = " ".join(parts[1:])
+    for k, (a, b, c, _) in grid.items():
+        lines[k] = f"{a} {b} {c}"
+    for k, (a, b, c, _) in rb.items():
+        lines[k] = f"{a} {b} {c}"
+    out.write_text("".join(f"{k} {v}\n" for k, v in sorted(lines.items())))
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--classes", default="32,64,128,256")
+    ap.add_argument("--classes", default="32,64,128,256", help="full-grid size classes ('' to skip)")
+    ap.add_argument("--rb-classes", default="32,64,128,256,512", help="red-black size classes ('' to skip)")
     ap.add_argument("--out", default=str(LIB_PATH.parent / "tuning_gfx950.txt"))
     ap.add_argument("--iters", type=int, default=12)
     a = ap.parse_args()
     t = time.time()
-    tune([int(c) for c in a.classes.split(",")], Path(a.out), a.iters)
+    grid = tune([int(c) for c in a.classes.split(",")], Path(a.out), a.iters) if a.classes else {}
+    rb = tune_rb([int(c) for c in a.rb_classes.split(",")], a.iters) if a.rb_classes else {}
+    write_table(Path(a.out), grid, rb)
     print(f"tuned in {time.time() - t:.0f} s -> {a.out}", flush=True)
     return 0
 

@@ -365,6 +365,15 @@ int fast_ctx_set_rb_shape(fast_ctx* ctx, int ty, int tz, int ahead) {
     return FAST_OK;
 }
 
+int fast_ctx_rb_shape(const fast_ctx* ctx, int nx, int ny, int nz, int* shape) {
+    if (!ctx || !shape || nx < 1 || ny < 1 || nz < 1) return FAST_E_BADARG;
+    const RbShape r = rb_shape(ctx, nx, ny, nz);
+    shape[0] = r.ty;
+    shape[1] = r.tz;
+    shape[2] = r.ahead;
+    return FAST_OK;
+}
+
 int fast_ctx_set_round(fast_ctx* ctx, int iterations) {
     if (!ctx || iterations < 0 || iterations > 1024) return FAST_E_BADARG;
     ctx->chunk = iterations ? iterations : 16;

@@ -148,6 +148,14 @@ class Context:
         tuned/automatic choice.  Results do not depend on it."""
         check(lib().fast_ctx_set_rb_shape(self._h, ty, tz, ahead), "fast_ctx_set_rb_shape", self._h)
 
+    def rb_shape(self, dims) -> tuple:
+        """(rows per tile, planes per tile, prefetch) a red-black solve of a volume of `dims`
+        (nx, ny, nz) runs with."""
+        out = (C.c_int * 3)()
+        check(lib().fast_ctx_rb_shape(self._h, int(dims[0]), int(dims[1]), int(dims[2]), out), "fast_ctx_rb_shape",
+              self._h)
+        return tuple(out)
+
     def volume(self, mask: np.ndarray) -> "Volume":
         return Volume(self, mask)
 

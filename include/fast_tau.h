@@ -133,6 +133,10 @@ int fast_ctx_set_tile_shape(fast_ctx* ctx, int tx, int ty, int tz);
  * "rb_" table entries, else 4 rows, the size rule and 2).  Used by the autotuner
  * (fast_amd/autotune.py).  No reference counterpart. */
 int fast_ctx_set_rb_shape(fast_ctx* ctx, int ty, int tz, int ahead);
+/* The red-black tile shape a solve of an nx x ny x nz volume would run with (forced, tuned or
+ * automatic): shape[0] = rows per tile, shape[1] = planes per tile, shape[2] = prefetch.
+ * No reference counterpart. */
+int fast_ctx_rb_shape(const fast_ctx* ctx, int nx, int ny, int nz, int* shape);
 /* Polling-round length in Krylov iterations (captured as one HIP graph); 0 = default 16. */
 int fast_ctx_set_round(fast_ctx* ctx, int iterations);
 
