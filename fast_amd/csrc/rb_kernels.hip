@@ -319,14 +319,14 @@ struct IC {
     static constexpr int value = N;
 };
 
-// Steps J .. R-1 of one turn of a register ring: f(IC<j>, t + j) for each j, leaving the
-// turn (true) after the step of plane `last`.
-template <int J, int R, class F>
+// Steps J .. N-1 of an unrolled stretch of a march over a register ring of length R:
+// f(IC<j mod R>, t + j) for each j, leaving the stretch (true) after the step of plane `last`.
+template <int J, int N, int R, class F>
 __device__ __forceinline__ bool ring_turn(F& f, int t, int last) {
-    if constexpr (J < R) {
-        f(IC<J>{}, t + J);
+    if constexpr (J < N) {
+        f(IC<J % R>{}, t + J);
         if (t + J >= last) return true;
-        return ring_turn<J + 1, R>(f, t, last);
+        return ring_turn<J + 1, N, R>(f, t, last);
     } else {
         return false;
     }
@@ -390,6 +390,12 @@ __global__ __launch_bounds__(kRbTX*(TY + 3)) void k_rb_yq(RbLaunch L) {
     // Element of the column in plane -1.  Outside the padded layout it is element 0, and a
     // plane offset outside -1 .. nz is 0: either way the load reads a ghost (+0, code 0).
     const unsigned cel = col_pad ? (unsigned)(kc + 1) + (unsigned)d.px * (unsigned)(yc + 1) : 0u;
+    // The codes are read as the 4-byte word holding the column's byte: plane pitches are
+    // multiples of 4 (rb_row_pitch), so the byte sits at the same shift in every plane, and a
+    // 32-bit register ring needs no re-extension of a byte between the steps of the march.
+    const unsigned cel4 = cel >> 2, csh = (cel & 3u) * 8u;
+    const unsigned* code_r4 = reinterpret_cast<const unsigned*>(code_r);
+    const unsigned* code_b4 = reinterpret_cast<const unsigned*>(code_b);
     const int pdst = has_col ? (kc - k0 + 2) + (yc - y0 + 2) * PW : PW * PH;
     const int ypos = (kc - k0 + 1) + (yc - y0 + 1) * YW;
     const int ydst = has_col ? ypos : YW * YH;
@@ -424,11 +430,7 @@ __global__ __launch_bounds__(kRbTX*(TY + 3)) void k_rb_yq(RbLaunch L) {
         }
     }
     // plane zz's offset in elements (uniform); 0 outside -1 .. nz
-    auto plane = [&](int zz) { return (zz >= -1 && zz <= d.nz) ? d.pp * (unsigned)(zz + 1) : 0u; };
-    auto ld_p = [&](int zz) { return ld(P + plane(zz), cel); };
-    auto ld_po = [&](int zz) { return ld(P + plane(zz), pel); };
-    auto ld_cb = [&](int zz) { return (int)ld(code_b + plane(zz), cel); };
-    auto ld_cr = [&](int zz) { return (int)ld(code_r + plane(zz), cel); };
+    auto plane = [&](int zz) { return (unsigned)(zz + 1) <= (unsigned)(d.nz + 1) ? d.pp * (unsigned)(zz + 1) : 0u; };
     // diagonal entry (and inverse) of a row: tables, or the formula on a y or z box face
     auto diag = [&](int code, int s, int yy, int zz, bool zface) {
         if (yface || zface) return diag_at<double>(tabs.tab, code, box_flags(2 * kc + s, yy, zz, d.nx, d.ny, d.nz), d.primary, c);
@@ -442,24 +444,28 @@ __global__ __launch_bounds__(kRbTX*(TY + 3)) void k_rb_yq(RbLaunch L) {
     // t+AHEAD-1)
     const int t0 = z0 - 1;
     float pr[R], po[R];
-    int cb[R], cr[R];
-    pr[R - 1] = ld_p(t0 - 1);
+    unsigned cb[R], cr[R];   // code words (the column's byte at shift csh)
+    pr[R - 1] = ld(P + plane(t0 - 1), cel);
     cr[R - 1] = 0;   // q(t0 - 1) is never formed
 #pragma unroll
     for (int i = 0; i < AHEAD; ++i) {
-        pr[i] = ld_p(t0 + i);
-        po[i] = ld_po(t0 + i);
-        cb[i] = ld_cb(t0 + i);
-        if (i < AHEAD - 1) cr[i] = ld_cr(t0 + i);
+        const unsigned o = plane(t0 + i);
+        pr[i] = ld(P + o, cel);
+        po[i] = ld(P + o, pel);
+        cb[i] = ld(code_b4 + (o >> 2), cel4);
+        if (i < AHEAD - 1) cr[i] = ld(code_r4 + (o >> 2), cel4);
     }
     double ya = 0.0, yb = 0.0;   // the column's Y of planes t-2 and t-1
     auto step = [&](auto J, int t) {
         constexpr int j = decltype(J)::value;
         constexpr int jm = (j + R - 1) % R, jn = (j + 1) % R;
-        pr[(j + AHEAD) % R] = ld_p(t + AHEAD);
-        po[(j + AHEAD) % R] = ld_po(t + AHEAD);
-        cb[(j + AHEAD) % R] = ld_cb(t + AHEAD);
-        cr[(j + AHEAD - 1) % R] = ld_cr(t + AHEAD - 1);
+        {
+            const unsigned o = plane(t + AHEAD), oq = plane(t + AHEAD - 1);
+            pr[(j + AHEAD) % R] = ld(P + o, cel);
+            po[(j + AHEAD) % R] = ld(P + o, pel);
+            cb[(j + AHEAD) % R] = ld(code_b4 + (o >> 2), cel4);
+            cr[(j + AHEAD - 1) % R] = ld(code_r4 + (oq >> 2), cel4);
+        }
         float* pb = pbuf[t & 1];
         pb[pdst] = pr[j];
         pb[podst] = po[j];
@@ -469,7 +475,7 @@ __global__ __launch_bounds__(kRbTX*(TY + 3)) void k_rb_yq(RbLaunch L) {
             const int s = sb0 ^ (t & 1);
             const float* pl = pb + pdst;
             const float u[6] = {pr[jm], pl[-PW], pl[s - 1], pl[s], pl[PW], pr[jn]};
-            const int code = cb[j];
+            const int code = (int)((cb[j] >> csh) & 0xffu);
             const DiagInv<double> di = diag(code, s, yc, t, t == 0 || t == d.nz - 1);
             const double yv = rb_sum(tabs.ftab, code, u) * di.iv;
             const bool ok = (s ? ok1 : ok0) && t >= 0 && t < d.nz;
@@ -481,7 +487,7 @@ __global__ __launch_bounds__(kRbTX*(TY + 3)) void k_rb_yq(RbLaunch L) {
             if (own && (sq ? ok1 : ok0) && tq >= z0 && tq < z1) {
                 const double* yl = ybuf[tq & 1] + ypos;
                 const double o[6] = {ya, yl[-YW], yl[sq - 1], yl[sq], yl[YW], yt};
-                const int code_q = cr[jm];
+                const int code_q = (int)((cr[jm] >> csh) & 0xffu);
                 const DiagInv<double> dq = diag(code_q, sq, yc, tq, tq == 0 || tq == d.nz - 1);
                 const double p64 = (double)pr[jm];
                 const double qv = fma(dq.d, p64, -rb_sum(tabs.ftab, code_q, o));
@@ -493,9 +499,10 @@ __global__ __launch_bounds__(kRbTX*(TY + 3)) void k_rb_yq(RbLaunch L) {
         }
     };
     // z0 - 1 .. z1: Y of the planes around the tile, q of its planes (the exits are uniform
-    // over the block)
+    // over the block).  Unrolling over several ring turns (fewer back edges, where the
+    // compiler waits for every load in flight) measured no faster.
     for (int t = t0;; t += R)
-        if (ring_turn<0, R>(step, t, z1)) break;
+        if (ring_turn<0, R, R>(step, t, z1)) break;
     block_reduce_store<AccDD, 1, NT>(acc, L.part + (long long)tile * kPart);
 }
 

@@ -228,7 +228,7 @@ int rebuild_rb(fast_server* s, const std::vector<Req*>& adm) {
         d.ny = r->ny;
         d.nz = r->nz;
         d.hx = (r->nx + 1) / 2;
-        d.px = d.hx + 2;
+        d.px = rb_row_pitch(d.hx);
         d.pp = (unsigned)d.px * (unsigned)(r->ny + 2);
         d.dir = r->prm.dir;
         d.primary = r->prm.dir / 2;
@@ -694,7 +694,7 @@ int fast_server_submit(fast_server* s, const uint8_t* mask, int nx, int ny, int 
     r->prm = *params;
     if (s->rb) {   // padded per-colour elements
         const int hx = (nx + 1) / 2;
-        r->elems = align_up((size_t)(hx + 2) * (size_t)(ny + 2) * (size_t)(nz + 2), kAlign);
+        r->elems = align_up((size_t)rb_row_pitch(hx) * (size_t)(ny + 2) * (size_t)(nz + 2), kAlign);
         r->tz = rb_shape(s->ctx, nx, ny, nz).tz;
     } else {
         r->elems = align_up(n, kAlign);

@@ -935,7 +935,7 @@ static int prepare_rb(RbBatch& B) {
         r.ny = v->ny;
         r.nz = v->nz;
         r.hx = (v->nx + 1) / 2;
-        r.px = r.hx + 2;
+        r.px = rb_row_pitch(r.hx);
         r.pp = (unsigned)r.px * (unsigned)(v->ny + 2);
         r.dir = j.prm.dir;
         r.primary = j.prm.dir / 2;
@@ -1175,7 +1175,7 @@ int solve_jobs(fast_ctx* ctx, std::vector<Job>& jobs) {
             // grouping.
             auto state_bytes = [](const Job* j) {
                 const fast_volume* v = j->vol;
-                return 46 * (size_t)((v->nx + 1) / 2 + 2) * (size_t)(v->ny + 2) * (size_t)(v->nz + 2);
+                return 46 * (size_t)rb_row_pitch((v->nx + 1) / 2) * (size_t)(v->ny + 2) * (size_t)(v->nz + 2);
             };
             bool all_large = ctx->group_bytes > 0;
             for (const Job* j : g.second) all_large = all_large && 2 * state_bytes(j) >= ctx->group_bytes;

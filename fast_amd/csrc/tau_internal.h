@@ -207,6 +207,11 @@ struct RbLaunch {
 
 constexpr int kRbTX = 64;   // threads per block along k (one wave per row of a tile)
 
+// Row pitch of the padded per-colour layout: at least hx + 2 (a ghost slot on either side of
+// a row) and a multiple of 4, so every plane pitch is too and a 4-byte word of a code array
+// holds the same four slots of a row in every plane.
+inline int rb_row_pitch(int hx) { return (hx + 2 + 3) & ~3; }
+
 // Diagonal tables and per-colour codes (after the vectors and codes were zero-filled).
 hipError_t rb_setup(const RbLaunch&, hipStream_t);
 // One CG iteration on S for the volumes in ST_RUNNING (4 passes, 2 per-volume finalizes).
