@@ -106,9 +106,10 @@ struct RbIn {
 // ghosts make neighbour reads safe, so each plane waits only for its own loads.
 template <int KIND, int TY>
 __global__ __launch_bounds__(kRbTX* TY) void k_rb_it(RbLaunch L) {
-    const int tile = xcd_remap(blockIdx.x, gridDim.x);
-    if (tile >= L.ntiles) return;
-    const int v = L.tile_vol[tile];
+    const int slot = xcd_remap(blockIdx.x, gridDim.x);
+    if (slot >= L.ntiles) return;
+    const int v = L.tile_vol[slot];
+    const int tile = L.tile_id ? L.tile_id[slot] : slot;   // the tile's index in the batch
     const RbDesc d = L.vol[v];   // by value: kept in registers across the stores
     const VolState<double>* vs = L.st + v;
     if (vs->status != ST_RUNNING) return;
@@ -213,9 +214,10 @@ __global__ __launch_bounds__(kRbTX* TY) void k_rb_q(RbLaunch L) {
     constexpr int W = kRbTX + 2, PL = W * (TY + 2);
     constexpr int NT = kRbTX * TY;
     constexpr int RND = (PL + NT - 1) / NT;
-    const int tile = xcd_remap(blockIdx.x, gridDim.x);
-    if (tile >= L.ntiles) return;
-    const int v = L.tile_vol[tile];
+    const int slot = xcd_remap(blockIdx.x, gridDim.x);
+    if (slot >= L.ntiles) return;
+    const int v = L.tile_vol[slot];
+    const int tile = L.tile_id ? L.tile_id[slot] : slot;   // the tile's index in the batch
     const RbDesc d = L.vol[v];
     const VolState<double>* vs = L.st + v;
     if (vs->status != ST_RUNNING) return;
@@ -358,9 +360,10 @@ __global__ __launch_bounds__(kRbTX*(TY + 3)) void k_rb_yq(RbLaunch L) {
     constexpr int YW = kRbTX + 2, YH = TY + 2;   // Y plane: slots k0-1 .. k0+64, rows y0-1 .. y0+TY
     constexpr int NPO = 2 * PW + 2 * (TY + 2) + 4;   // p points that no column owns
     static_assert(kRbTX == 64 && NPO <= 3 * 64, "one p-only point per lane of waves TY .. TY+2");
-    const int tile = xcd_remap(blockIdx.x, gridDim.x);
-    if (tile >= L.ntiles) return;
-    const int v = L.tile_vol[tile];
+    const int slot = xcd_remap(blockIdx.x, gridDim.x);
+    if (slot >= L.ntiles) return;
+    const int v = L.tile_vol[slot];
+    const int tile = L.tile_id ? L.tile_id[slot] : slot;   // the tile's index in the batch
     const RbDesc d = L.vol[v];
     const VolState<double>* vs = L.st + v;
     if (vs->status != ST_RUNNING) return;
@@ -517,9 +520,10 @@ __global__ __launch_bounds__(kRbTX*(TY + 3)) void k_rb_yq(RbLaunch L) {
 // primary coordinate: Y64 evaluates it for the red neighbours, R64 stores it.
 template <int KIND, int TY>
 __global__ __launch_bounds__(kRbTX* TY) void k_rb(RbLaunch L) {
-    const int tile = xcd_remap(blockIdx.x, gridDim.x);
-    if (tile >= L.ntiles) return;
-    const int v = L.tile_vol[tile];
+    const int slot = xcd_remap(blockIdx.x, gridDim.x);
+    if (slot >= L.ntiles) return;
+    const int v = L.tile_vol[slot];
+    const int tile = L.tile_id ? L.tile_id[slot] : slot;   // the tile's index in the batch
     const RbDesc d = L.vol[v];
     VolState<double>* vs = L.st + v;
     if (KIND == RB_UPD || KIND == RB_Y64 || KIND == RB_R64) {

@@ -275,6 +275,7 @@ int rebuild_rb(fast_server* s, const std::vector<Req*>& adm) {
     L.ahead = s->rbs.ahead;
     L.vol = s->d_rb;
     L.tile_vol = s->d_tile;
+    L.tile_id = nullptr;   // tiles are listed in batch order
     L.coef = s->d_coef;
     L.st = s->d_st;
     L.part = s->d_part;

@@ -670,8 +670,10 @@ static int prepare(Batch<T, A>& B) {
 // Poll the device-resident states every `chunk` iterations, keeping one chunk in flight
 // (no host bubble).  `step` enqueues one chunk.  Returns when every volume reached a final
 // status (`final_status`); restarts are served synchronously (they are rare).
-template <class BT, class Step, class Restart>
-static int drive(BT& B, int final_status, long max_chunks, Step step, Restart restart) {
+// `poll` sees each snapshot of the states while the next chunk is in flight; what it enqueues
+// runs after that chunk.
+template <class BT, class Step, class Restart, class Poll>
+static int drive(BT& B, int final_status, long max_chunks, Step step, Restart restart, Poll poll) {
     using St = typename decltype(B.hst)::value_type;
     fast_ctx* ctx = B.ctx;
     const int nv = (int)B.jobs.size();
@@ -719,6 +721,8 @@ static int drive(BT& B, int final_status, long max_chunks, Step step, Restart re
                 slot ^= 1;
                 continue;
             }
+            e = poll(snap[prev]);
+            if (!hip_ok(ctx, e, "poll step")) return FAST_E_HIP;
         }
         if (chunks > max_chunks) {
             set_error(ctx, "solve did not terminate within %ld polling rounds", max_chunks);
@@ -730,6 +734,11 @@ static int drive(BT& B, int final_status, long max_chunks, Step step, Restart re
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e == hipSuccess) e = hipMemcpy(B.hst.data(), B.L.st, sbytes, hipMemcpyDeviceToHost);
     return hip_ok(ctx, e, "final state") ? FAST_OK : FAST_E_HIP;
+}
+
+template <class BT, class Step, class Restart>
+static int drive(BT& B, int final_status, long max_chunks, Step step, Restart restart) {
+    return drive(B, final_status, max_chunks, step, restart, [](const void*) { return hipSuccess; });
 }
 
 // Flux integral on the device, then tau = porosity / (dc * N_primary) in T like the
@@ -891,6 +900,8 @@ struct RbBatch {
     double* plane_out = nullptr;
     int* plane_vol = nullptr;
     int* plane_k = nullptr;
+    int* tile_vol_c = nullptr;   // compacted launch: volume and batch index of each live tile
+    int* tile_id_c = nullptr;
     hipGraphExec_t gexec = nullptr;
 
     ~RbBatch() {
@@ -1005,6 +1016,10 @@ static int prepare_rb(RbBatch& B) {
     o = align_up(o + sizeof(VolState<double>) * nv, 256);
     const size_t o_tile = o;
     o = align_up(o + sizeof(int) * tiles, 256);
+    const size_t o_tvc = o;
+    o = align_up(o + sizeof(int) * tiles, 256);
+    const size_t o_tidc = o;
+    o = align_up(o + sizeof(int) * tiles, 256);
     const size_t o_part = o;
     o = align_up(o + sizeof(double) * 8 * (size_t)tiles, 256);
     const size_t o_partb = o;
@@ -1048,6 +1063,9 @@ static int prepare_rb(RbBatch& B) {
     L.ahead = bs.ahead;
     L.vol = (const RbDesc*)(m + o_rb);
     L.tile_vol = (const int*)(m + o_tile);
+    L.tile_id = nullptr;
+    B.tile_vol_c = (int*)(m + o_tvc);
+    B.tile_id_c = (int*)(m + o_tidc);
     L.coef = (const Coef<double>*)(m + o_coef);
     L.st = (VolState<double>*)(m + o_st);
     L.part = (double*)(m + o_part);
@@ -1108,7 +1126,37 @@ static int solve_mixed_rb(fast_ctx* ctx, std::vector<Job*>& jobs) {
     };
     auto step = [&]() -> hipError_t { return replay(B, round); };
     auto restart = [&]() -> hipError_t { return hipSuccess; };
-    rc = drive(B, ST_IDLE, max_chunks, step, restart);
+    // Batch tails: once the unfinished volumes hold at most half of the launched tiles, later
+    // rounds launch only theirs (a captured round is replaced by a new capture).
+    const RbLaunch full = B.L;
+    auto poll = [&](const VolState<double>* snap) -> hipError_t {
+        long live = 0;
+        for (size_t i = 0; i < jobs.size(); ++i)
+            if (snap[i].status != ST_IDLE) live += B.hrb[i].ntiles;
+        if (live == 0 || 2 * live > B.L.ntiles || B.L.ntiles < kRbCompactMin) return hipSuccess;
+        int* h = (int*)pinned(ctx, 2, 2 * sizeof(int) * (size_t)live);
+        if (!h) return hipErrorOutOfMemory;
+        long k = 0;
+        for (size_t i = 0; i < jobs.size(); ++i) {
+            if (snap[i].status == ST_IDLE) continue;
+            for (int t = 0; t < B.hrb[i].ntiles; ++t, ++k) {
+                h[k] = (int)i;
+                h[live + k] = B.hrb[i].tile_base + t;
+            }
+        }
+        hipError_t e2 = hipMemcpyAsync(B.tile_vol_c, h, sizeof(int) * live, hipMemcpyHostToDevice, s);
+        if (e2 == hipSuccess) e2 = hipMemcpyAsync(B.tile_id_c, h + live, sizeof(int) * live, hipMemcpyHostToDevice, s);
+        B.L.tile_vol = B.tile_vol_c;
+        B.L.tile_id = B.tile_id_c;
+        B.L.ntiles = (int)live;
+        if (B.gexec) {
+            (void)hipGraphExecDestroy(B.gexec);
+            B.gexec = nullptr;
+        }
+        return e2;
+    };
+    rc = drive(B, ST_IDLE, max_chunks, step, restart, poll);
+    B.L = full;   // the scatter covers every volume
     if (rc) return rc;
     e = ft::rb_finish(B.L, s);
     if (!hip_ok(ctx, e, "scatter")) return FAST_E_HIP;

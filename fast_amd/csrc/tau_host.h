@@ -21,6 +21,10 @@ constexpr int kMixedUpdateEvery = 4;
 // vector (8 bytes per voxel) must stay below 4 GiB.
 constexpr size_t kMaxSolveVoxels = ((size_t)1 << 29) - 1;
 
+// Red-black batches launch only the tiles of unfinished volumes once those are at most half
+// of a launch of at least this many tiles.
+constexpr int kRbCompactMin = 512;
+
 struct Shape {
     int tx, ty, tz;
 };

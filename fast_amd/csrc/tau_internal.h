@@ -189,7 +189,8 @@ struct RbLaunch {
     int staged;       // 2: Y and Q fused (no Y vector); 1: Q pass with the Y planes staged in LDS; 0: six
                       // Y loads per voxel
     const RbDesc* vol;
-    const int* tile_vol;
+    const int* tile_vol;   // volume of each launched tile
+    const int* tile_id;    // its index in the batch (tile_base + position); null: the launch index
     const Coef<double>* coef;
     VolState<double>* st;
     double* part;     // ntiles * 8: red-voxel partial sums

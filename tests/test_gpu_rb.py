@@ -144,6 +144,29 @@ def test_y_and_q_forms_bit_identical(ctx, monkeypatch):
         vol.close()
 
 
+def test_compacted_batch_equals_solo(ctx):
+    """Volumes that converge at different iterations: once the unfinished ones hold at most
+    half of the launched tiles, later rounds launch only theirs (a new graph is captured).
+    Every result stays bit-identical to its solo solve."""
+    vols, ps = [], []
+    for i in range(14):   # nx, ny, nz = 24, 40, 96: flow along x (24 planes) converges far sooner than along z
+        m = F.generate_spheres((24, 40, 96), 2.5 + 0.25 * (i % 7), 0.4 + 0.02 * (i % 7), 70 + i)
+        vols.append(ctx.volume(m))
+        ps.append(T.make_params(0 if i % 2 == 0 else 4, CG, "mixed", 1e-9, 20000))
+    solo = [v.solve(p)[0] for v, p in zip(vols, ps)]
+    fast = max(r.iterations for i, r in enumerate(solo) if i % 2 == 0)
+    slow = min(r.iterations for i, r in enumerate(solo) if i % 2 == 1)
+    # the x half finishes first, several 16-iteration polling rounds before the z half: the
+    # batch has a tail, and half the tiles are left to launch
+    assert slow > fast + 48, (fast, slow)
+    batch = T.solve_batch(ctx, vols, ps)
+    for b, r in zip(batch, solo):
+        assert b.status == 0 and r.status == 0
+        assert b.tau == r.tau and b.iterations == r.iterations
+    for v in vols:
+        v.close()
+
+
 def test_tile_shapes_bit_identical(ctx):
     """Rows per tile (2, 4, 8), planes per tile and the fused pass's prefetch distance change
     only the launch geometry: every dot product is rounded once from a double-double sum, so
