@@ -104,7 +104,7 @@ struct RbIn {
 // Loads are issued two planes ahead of use into three register sets used in rotation (the loop
 // is unrolled by 3), all unconditional: an invalid point reads the first interior slot, and
 // ghosts make neighbour reads safe, so each plane waits only for its own loads.
-template <int KIND, int TY>
+template <int KIND, int TY, int AH>
 __global__ __launch_bounds__(kRbTX* TY) void k_rb_it(RbLaunch L) {
     const int slot = xcd_remap(blockIdx.x, gridDim.x);
     if (slot >= L.ntiles) return;
@@ -183,18 +183,19 @@ __global__ __launch_bounds__(kRbTX* TY) void k_rb_it(RbLaunch L) {
             acc[1].add((double)rs, di.iv * (double)rs);
         }
     };
-    RbIn i0, i1, i2;
-    load(ps.z0, i0);
-    load(ps.z0 + 1, i1);
-    // no early exit: a step past the last plane loads the first slot and does nothing, which
-    // keeps the loop body straight (the compiler then waits for each set's own loads only)
-    for (int z = ps.z0; z < ps.z1; z += 3) {
-        load(z + 2, i2);
-        proc(z, i0);
-        load(z + 3, i0);
-        proc(z + 1, i1);
-        load(z + 4, i1);
-        proc(z + 2, i2);
+    // loads AH planes ahead into a ring of AH + 1 register sets; no early exit: a step past
+    // the last plane loads the first slot and does nothing, which keeps the loop body straight
+    // (the compiler then waits for each set's own loads only)
+    constexpr int R = AH + 1;
+    RbIn in[R];
+#pragma unroll
+    for (int i = 0; i < AH; ++i) load(ps.z0 + i, in[i]);
+    for (int z = ps.z0; z < ps.z1; z += R) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            load(z + j + AH, in[(j + AH) % R]);
+            proc(z + j, in[j]);
+        }
     }
     if (KIND == RB_Q) block_reduce_store<AccDD, 1, kRbTX * TY>(reinterpret_cast<AccDD(&)[1]>(acc), L.part + (long long)tile * kPart);
     if (KIND == RB_U) block_reduce_store<AccDD, 2, kRbTX * TY>(acc, L.part + (long long)tile * kPart);
@@ -668,9 +669,12 @@ __global__ __launch_bounds__(256) void k_rb_tables(const RbDesc* vol, const Coef
 // Rows per tile (RbLaunch::ty) as a template argument: every kernel of a batch uses the same
 // tiles, so the partial sums of a pass line up with the tile ranges the finalizes reduce.
 #define FT_RB_TY(F)   \
-    switch (L.ty) {   \
-        case 2: F(2); \
-        case 8: F(8); \
+    switch (L.ty) {    \
+        case 2: F(2);  \
+        case 8: F(8);  \
+        case 10: F(10); \
+        case 12: F(12); \
+        case 13: F(13); \
         default: F(4); \
     }
 
@@ -683,10 +687,15 @@ void rb_launch(const RbLaunch& L, hipStream_t s) {
 #undef FT_GO
 }
 
+// Planes the streaming passes P and U load ahead.  3, 4 and 6 measured no faster than 2 on
+// MI355X (256^3 and 512^3), so every plane-marching pass loads two ahead.
+constexpr int kRbStreamAhead = 2;
+
 template <int KIND>
 void rb_launch_it(const RbLaunch& L, hipStream_t s) {
-#define FT_GO(TYV)                                                                                 \
-    hipLaunchKernelGGL((k_rb_it<KIND, TYV>), dim3(L.ntiles), dim3(kRbTX * TYV), 0, s, L); \
+    constexpr int AH = (KIND == RB_P || KIND == RB_U) ? kRbStreamAhead : 2;
+#define FT_GO(TYV)                                                                                     \
+    hipLaunchKernelGGL((k_rb_it<KIND, TYV, AH>), dim3(L.ntiles), dim3(kRbTX * TYV), 0, s, L); \
     return;
     FT_RB_TY(FT_GO)
 #undef FT_GO
@@ -752,7 +761,7 @@ static void rb_launch_q(const RbLaunch& L, hipStream_t s) {
 
 hipError_t rb_iter(const RbLaunch& L, hipStream_t s) {
     rb_launch_it<RB_P>(L, s);
-    if (L.staged != 2) rb_launch_it<RB_Y>(L, s);   // the fused pass computes Y itself
+    if (L.staged < 2) rb_launch_it<RB_Y>(L, s);   // the fused form computes Y itself
     rb_launch_q(L, s);
     hipLaunchKernelGGL((k_rb_fin<F_CG1>), dim3(L.nvol), dim3(kFinThreads), 0, s, L);
     rb_launch_it<RB_U>(L, s);
@@ -772,7 +781,7 @@ hipError_t rb_pass_only(const RbLaunch& L, int k, hipStream_t s) {
     switch (k) {
         case 0: rb_launch_it<RB_P>(L, s); break;
         case 1:
-            if (L.staged != 2) rb_launch_it<RB_Y>(L, s);
+            if (L.staged < 2) rb_launch_it<RB_Y>(L, s);
             break;
         case 2: rb_launch_q(L, s); break;
         case 3: rb_launch_it<RB_U>(L, s); break;
