@@ -400,16 +400,16 @@ def main() -> int:
     fused = os.environ.get("FAST_TAU_RB_STAGED", "2") == "2"
     if fused:   # Y computed inside the Q pass: p, red and black codes in, q out
         kernels = {
-            "k_rb_it<0> (P, red: p = D^-1 r + beta p)": (rb_pass[0], (4 + 4 + 1 + 4) * n_r),
+            "k_rb_it<0> (P, red: p = D^-1 r + beta p, x += alpha p_old)": (rb_pass[0], (4 + 4 + 1 + 4 + 2 * 4) * n_r),
             "k_rb_yq (Y+Q fused, red: q = S p = D_r p - A_rb D_b^-1 A_br p, p.q)": (rb_pass[2], (4 + 1 + 4) * n_r + 1 * n_b),
-            "k_rb_it<3> (U, red: x += alpha p, r -= alpha q, r.r, r.D^-1 r)": (rb_pass[3], (4 * 4 + 1 + 2 * 4) * n_r),
+            "k_rb_it<3> (U, red: r -= alpha q, r.r, r.D^-1 r)": (rb_pass[3], (2 * 4 + 1 + 4) * n_r),
         }
     else:
         kernels = {
-            "k_rb_it<0> (P, red: p = D^-1 r + beta p)": (rb_pass[0], (4 + 4 + 1 + 4) * n_r),
+            "k_rb_it<0> (P, red: p = D^-1 r + beta p, x += alpha p_old)": (rb_pass[0], (4 + 4 + 1 + 4 + 2 * 4) * n_r),
             "k_rb_it<1> (Y, black: Y = D_b^-1 A_br p)": (rb_pass[1], (1 + 8) * n_b + 4 * n_r),
             "k_rb_q (Q, red: q = S p = D_r p - A_rb Y, p.q)": (rb_pass[2], (1 + 4 + 4) * n_r + 8 * n_b),
-            "k_rb_it<3> (U, red: x += alpha p, r -= alpha q, r.r, r.D^-1 r)": (rb_pass[3], (4 * 4 + 1 + 2 * 4) * n_r),
+            "k_rb_it<3> (U, red: r -= alpha q, r.r, r.D^-1 r)": (rb_pass[3], (2 * 4 + 1 + 4) * n_r),
         }
     name, (kms, kbytes) = max(kernels.items(), key=lambda kv: kv[1][0])
     ach = kbytes / (kms * 1e-3) / 1e9
@@ -432,10 +432,10 @@ def main() -> int:
         nb = 512 ** 3
         nr5 = nb // 2
         if fused:
-            b5 = {0: ("k_rb_it<0> (P)", 13 * nr5), 2: ("k_rb_yq (Y+Q)", 10 * nr5), 3: ("k_rb_it<3> (U)", 25 * nr5)}
+            b5 = {0: ("k_rb_it<0> (P)", 21 * nr5), 2: ("k_rb_yq (Y+Q)", 10 * nr5), 3: ("k_rb_it<3> (U)", 13 * nr5)}
         else:
-            b5 = {0: ("k_rb_it<0> (P)", 13 * nr5), 1: ("k_rb_it<1> (Y)", 13 * nr5), 2: ("k_rb_q (Q)", 17 * nr5),
-                  3: ("k_rb_it<3> (U)", 25 * nr5)}
+            b5 = {0: ("k_rb_it<0> (P)", 21 * nr5), 1: ("k_rb_it<1> (Y)", 13 * nr5), 2: ("k_rb_q (Q)", 17 * nr5),
+                  3: ("k_rb_it<3> (U)", 13 * nr5)}
         dram = {"volume": "512^3 needles a=12 r=0.1 solid 0.55 seed 0, X_POS",
                 "red_black_tile": dict(zip(("rows", "planes", "prefetch_planes"), ctx.rb_shape((512, 512, 512)))),
                 "kernels": {k: {"avg_launch_ms": rb[i], "bytes_per_launch": b,

@@ -97,13 +97,18 @@ struct RbIn {
 };
 
 // The iteration passes, one thread per slot of a row, marching tz planes:
-//   P (red):   p = D^-1 r (first iteration) or D^-1 r + beta p_old
+//   P (red):   p = D^-1 r (first iteration) or D^-1 r + beta p_old; and, while x lacks the
+//              last iteration's step (xpend), x += alpha p_old with that iteration's alpha
 //   Y (black): Y = D_b^-1 (A_br p)
 //   Q (red):   q = D_r p - A_rb Y = S p; partial p.q
-//   U (red):   x += alpha p; r -= alpha q; partials r.r and r.(D^-1 r)
-// Loads are issued two planes ahead of use into three register sets used in rotation (the loop
-// is unrolled by 3), all unconditional: an invalid point reads the first interior slot, and
-// ghosts make neighbour reads safe, so each plane waits only for its own loads.
+//   U (red):   r -= alpha q; partials r.r and r.(D^-1 r)
+// x += alpha p is deferred from U to the next P pass, which loads p_old anyway: U then reads
+// r, q and the code (13 B per red voxel instead of 25) and P also reads and writes x (21
+// instead of 13).  x takes the same values and roundings; the update pass adds a pending
+// step before it folds x into x64.
+// Loads are issued AH planes ahead of use into AH + 1 register sets used in rotation (the
+// loop is unrolled by AH + 1), all unconditional: an invalid point reads the first interior
+// slot, and ghosts make neighbour reads safe, so each plane waits only for its own loads.
 template <int KIND, int TY, int AH>
 __global__ __launch_bounds__(kRbTX* TY) void k_rb_it(RbLaunch L) {
     const int slot = xcd_remap(blockIdx.x, gridDim.x);
@@ -120,6 +125,7 @@ __global__ __launch_bounds__(kRbTX* TY) void k_rb_it(RbLaunch L) {
     constexpr int kColour = (KIND == RB_Y) ? 1 : 0;
     const double alpha = vs->alpha, beta = vs->beta;
     const bool first = vs->first != 0;
+    const bool xpend = KIND == RB_P && vs->xpend != 0;   // uniform over the volume
     const bool odd = (vs->phase & 1) != 0;
     const float* p_old = L.p[odd ? 1 : 0] + d.off_c;
     float* p_new = L.p[odd ? 0 : 1] + d.off_c;
@@ -141,14 +147,13 @@ __global__ __launch_bounds__(kRbTX* TY) void k_rb_it(RbLaunch L) {
         if (KIND == RB_P) {
             in.a = ld(Rr, ci);
             in.b = ld(p_old, ci);   // unused on the first iteration
+            if (xpend) in.c = ld(X, ci);
         } else if (KIND == RB_Y) {
             rb_nbrs(p_new, d, ci, s, in.u);
         } else if (KIND == RB_Q) {
             in.a = ld(p_new, ci);
             rb_nbrs(Y, d, ci, s, in.w);
         } else {
-            in.a = ld(p_new, ci);
-            in.b = ld(X, ci);
             in.c = ld(Rr, ci);
             in.e = ld(Q, ci);
         }
@@ -167,6 +172,7 @@ __global__ __launch_bounds__(kRbTX* TY) void k_rb_it(RbLaunch L) {
             const double z_ = di.iv * (double)in.a;
             const double pn = first ? z_ : z_ + beta * (double)in.b;
             st(p_new, ci, (float)pn);
+            if (xpend) st(X, ci, (float)((double)in.c + alpha * (double)in.b));
         } else if (KIND == RB_Y) {
             st(Y, ci, rb_sum(tabs.ftab, code, in.u) * di.iv);
         } else if (KIND == RB_Q) {
@@ -175,8 +181,6 @@ __global__ __launch_bounds__(kRbTX* TY) void k_rb_it(RbLaunch L) {
             st(Q, ci, (float)qv);
             acc[0].add(pv, qv);
         } else {
-            const double pv = (double)in.a;
-            st(X, ci, (float)((double)in.b + alpha * pv));
             const float rs = (float)((double)in.c - alpha * (double)in.e);
             st(Rr, ci, rs);
             acc[0].add(rs, rs);
@@ -512,7 +516,8 @@ __global__ __launch_bounds__(kRbTX*(TY + 3)) void k_rb_yq(RbLaunch L) {
 
 // The passes outside the Krylov loop, one thread per slot of a row, marching tz planes:
 //   CODE:    per-colour stencil codes from the full-grid codes (set-up)
-//   UPD:     x64 += x (kept unless discarded), x = 0                      [update / fresh start]
+//   UPD:     x64 += x (kept unless discarded; a pending x += alpha p added first), x = 0
+//                                                                          [update / fresh start]
 //   Y64:     black x64b = D_b^-1 (b_b - A_br x64), partial |b_b|^2       [update / fresh start]
 //   R64:     red r = b_S - S x64 = (b_r - D_r x64) - A_rb x64b in FP64, stored in FP32;
 //            partials |r64|^2, |b_r|^2, |r32|^2, r32.(D^-1 r32)           [update / fresh start]
@@ -539,6 +544,11 @@ __global__ __launch_bounds__(kRbTX* TY) void k_rb(RbLaunch L) {
     constexpr int kColour = (KIND == RB_Y64) ? 1 : 0;
     const bool first_res = vs->refinements == 0;   // x64 not yet set: use x0
     const bool keep = vs->refinements > 0 && !vs->discard;
+    // x lacks alpha p of the last iteration (p is the newer buffer: the iteration's finalize
+    // has advanced the phase), added with the P pass's rounding
+    const bool xpend = KIND == RB_UPD && vs->xpend != 0;
+    const double alpha = vs->alpha;
+    const float* p_last = L.p[(vs->phase & 1) ? 1 : 0] + d.off_c;
     const int nprim = d.primary == 0 ? d.nx : (d.primary == 1 ? d.ny : d.nz);
     float* X = L.x + d.off_c;
     float* Rr = L.r + d.off_c;
@@ -568,7 +578,10 @@ __global__ __launch_bounds__(kRbTX* TY) void k_rb(RbLaunch L) {
             }
             if (x >= d.nx) continue;
             if (KIND == RB_UPD) {
-                if (keep) X64[ci] = X64[ci] + (double)X[ci];
+                if (keep) {
+                    const float xv = xpend ? (float)((double)X[ci] + alpha * (double)p_last[ci]) : X[ci];
+                    X64[ci] = X64[ci] + (double)xv;
+                }
                 X[ci] = 0.0f;
                 continue;
             }
@@ -637,6 +650,7 @@ __global__ __launch_bounds__(kFinThreads) void k_rb_fin(RbLaunch L) {
         reduce_volume<1, kFinThreads>(L.part_b, d.tile_base, d.ntiles, totb);
         if (threadIdx.x == 0) {
             tot[1] += totb[0];   // |b|^2 over both colours
+            s->xpend = 0;        // the update pass added it
             mixres_update(s, tot);
         }
         return;
@@ -649,6 +663,8 @@ __global__ __launch_bounds__(kFinThreads) void k_rb_fin(RbLaunch L) {
         double ta[3] = {0.0, 0.0, 0.0};
 #pragma unroll
         for (int k = 0; k < NP; ++k) ta[k] = tot[k];
+        // after Q: the P pass has added the pending step; after U: x lacks this iteration's
+        s->xpend = FIN == F_CG2 ? 1 : 0;
         fin_update<double, FIN>(s, ta);
     }
 }
