@@ -8,7 +8,7 @@ compiled block shape (tx x ty) and a range of tz, and writes the fastest as line
 
 Red-black mixed CG (the default mixed CG): for every size class it times one red-black
 iteration (the P, fused Y+Q and U passes with both finalizes) for every rows-per-tile (2, 4,
-8, 10, 12, 13), planes per tile and fused-pass prefetch distance (2, 3), and writes the fastest as lines
+8, 9, 10, 12, 13), planes per tile and fused-pass prefetch distance (2, 3), and writes the fastest as lines
 "rb_<x>_<y>_<z> <ty> <tz> <ahead>".
 
 The lines go to tuning_gfx950.txt next to libfasttau.so, where the library picks them up
@@ -55,7 +55,7 @@ def time_iteration(ctx, vol, precision: int, solver: int, iters: int) -> float:
     return ms[3]
 
 
-RB_TY = (2, 4, 8, 10, 12, 13)
+RB_TY = (2, 4, 8, 9, 10, 12, 13)
 RB_TZ = (4, 8, 12, 16, 24, 32, 48, 64)
 RB_AHEAD = (2, 3)
 

@@ -349,15 +349,20 @@ __device__ __forceinline__ bool ring_turn(F& f, int t, int last) {
 // ahead, the march unrolled by the ring length so no register is copied while its load is in
 // flight) and computes the black Y of its column; the own rows also form q.  The x and y
 // neighbours come from LDS: a p plane over slots k0-2 .. k0+65 and rows y0-2 .. y0+TY+1
-// (the points no column owns are loaded by the lanes of waves TY .. TY+2, one each) and a Y
-// plane over slots k0-1 .. k0+64 and rows y0-1 .. y0+TY, both double-buffered.  Step t writes
-// p(t), passes one barrier, computes Y(t) and q(t-1): a buffer is rewritten two steps after it
-// was read, with a barrier in between.  Y and q take the same values, roundings and summation
-// order as k_rb_it<RB_Y> followed by k_rb_it<RB_Q>, so results are bit-identical.  Per red
-// voxel the pass reads p and the two codes and writes q (10 bytes, against 30 for the separate
-// passes, whose Y is FP64); the halo's Y is computed by both blocks that read it.
+// (the points no column owns are loaded by the lanes of waves TY .. TY+2, one each),
+// double-buffered, and a Y plane over slots k0-1 .. k0+64 and rows y0-1 .. y0+TY in a ring of
+// R buffers indexed like the registers.  Step t writes p(t), passes one barrier, computes Y(t)
+// and q(t-1), whose z neighbours Y(t-2) and Y(t) are the column's own (Y(t-2) read back from
+// the ring rather than kept in registers); a buffer is rewritten two steps or more after it
+// was last read, with a barrier in between.  Y and q take the same values, roundings and
+// summation order as k_rb_it<RB_Y> followed by k_rb_it<RB_Q>, so results are bit-identical.
+// Per red voxel the pass reads p and the two codes and writes q (10 bytes, against 30 for the
+// separate passes, whose Y is FP64); the halo's Y is computed by both blocks that read it.
+// With a prefetch of 2 planes the pass fits 80 VGPRs, 6 waves per SIMD: two blocks of up to
+// 12 waves (9 rows) per CU.  At 84 VGPRs (5 waves per SIMD) a CU held one block of 8 to 13 rows.
 template <int TY, int AHEAD>
-__global__ __launch_bounds__(kRbTX*(TY + 3)) void k_rb_yq(RbLaunch L) {
+__global__ __launch_bounds__(kRbTX*(TY + 3)) __attribute__((amdgpu_waves_per_eu(AHEAD == 2 ? 6 : 1)))
+void k_rb_yq(RbLaunch L) {
     static_assert(AHEAD >= 2, "p of plane t+1 is needed at step t");
     constexpr int R = AHEAD + 2;   // ring length: planes t-1 .. t+AHEAD
     constexpr int NT = kRbTX * (TY + 3);
@@ -374,7 +379,7 @@ __global__ __launch_bounds__(kRbTX*(TY + 3)) void k_rb_yq(RbLaunch L) {
     if (vs->status != ST_RUNNING) return;
     __shared__ RbTabs tabs;
     __shared__ float pbuf[2][PW * PH + 1];    // + a spare slot for lanes with nothing to store
-    __shared__ double ybuf[2][YW * YH + 1];
+    __shared__ double ybuf[R][YW * YH + 1];
     const Coef<double> c = L.coef[v];
     rb_load_tables<2, NT>(L, v, c, tabs);   // Y reads D_b^-1, Q reads D_r
     const int lt = tile - d.tile_base;
@@ -463,7 +468,6 @@ __global__ __launch_bounds__(kRbTX*(TY + 3)) void k_rb_yq(RbLaunch L) {
         cb[i] = ld(code_b4 + (o >> 2), cel4);
         if (i < AHEAD - 1) cr[i] = ld(code_r4 + (o >> 2), cel4);
     }
-    double ya = 0.0, yb = 0.0;   // the column's Y of planes t-2 and t-1
     auto step = [&](auto J, int t) {
         constexpr int j = decltype(J)::value;
         constexpr int jm = (j + R - 1) % R, jn = (j + 1) % R;
@@ -488,13 +492,13 @@ __global__ __launch_bounds__(kRbTX*(TY + 3)) void k_rb_yq(RbLaunch L) {
             const double yv = rb_sum(tabs.ftab, code, u) * di.iv;
             const bool ok = (s ? ok1 : ok0) && t >= 0 && t < d.nz;
             const double yt = ok ? yv : 0.0;
-            ybuf[t & 1][ydst] = yt;
+            ybuf[j][ydst] = yt;
             // q(t - 1) of the column's red voxel (own rows)
             const int tq = t - 1;
             const int sq = s;   // (yc + t - 1) & 1: the red voxel of plane t-1 has the black's parity of plane t
             if (own && (sq ? ok1 : ok0) && tq >= z0 && tq < z1) {
-                const double* yl = ybuf[tq & 1] + ypos;
-                const double o[6] = {ya, yl[-YW], yl[sq - 1], yl[sq], yl[YW], yt};
+                const double* yl = ybuf[jm] + ypos;
+                const double o[6] = {ybuf[(j + R - 2) % R][ypos], yl[-YW], yl[sq - 1], yl[sq], yl[YW], yt};
                 const int code_q = (int)((cr[jm] >> csh) & 0xffu);
                 const DiagInv<double> dq = diag(code_q, sq, yc, tq, tq == 0 || tq == d.nz - 1);
                 const double p64 = (double)pr[jm];
@@ -502,8 +506,6 @@ __global__ __launch_bounds__(kRbTX*(TY + 3)) void k_rb_yq(RbLaunch L) {
                 st(Q + d.pp * (unsigned)(tq + 1), cel, (float)qv);
                 acc[0].add(p64, qv);
             }
-            ya = yb;
-            yb = yt;
         }
     };
     // z0 - 1 .. z1: Y of the planes around the tile, q of its planes (the exits are uniform
@@ -688,6 +690,7 @@ __global__ __launch_bounds__(256) void k_rb_tables(const RbDesc* vol, const Coef
     switch (L.ty) {    \
         case 2: F(2);  \
         case 8: F(8);  \
+        case 9: F(9);  \
         case 10: F(10); \
         case 12: F(12); \
         case 13: F(13); \

@@ -229,7 +229,7 @@ std::string rb_key(int nx, int ny, int nz) {
     return b;
 }
 
-bool rb_ty_compiled(int ty) { return ty == 2 || ty == 4 || ty == 8 || ty == 10 || ty == 12 || ty == 13; }
+bool rb_ty_compiled(int ty) { return ty == 2 || ty == 4 || ty == 8 || ty == 9 || ty == 10 || ty == 12 || ty == 13; }
 
 int rb_tz(int hx, int ny, int nz, int ty) {
     const long long txy = (long long)((hx + kRbTX - 1) / kRbTX) * ((ny + ty - 1) / ty);

@@ -186,7 +186,7 @@ struct RbDesc {
 
 struct RbLaunch {
     int ntiles, nvol;
-    int ty;           // rows per tile (2, 4, 8, 10, 12 or 13) of every red-black pass of this launch set
+    int ty;           // rows per tile (2, 4, 8, 9, 10, 12 or 13) of every red-black pass of this launch set
     int ahead;        // fused Y+Q pass: planes its loads run ahead (2 or 3)
     int staged;       // 2: Y and Q fused (no Y vector); 1: Q pass with the Y planes staged in LDS; 0: six
                       // Y loads per voxel

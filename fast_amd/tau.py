@@ -143,7 +143,7 @@ class Context:
         return lib().fast_ctx_device_bytes(self._h)
 
     def set_rb_shape(self, ty: int = 0, tz: int = 0, ahead: int = 2):
-        """Force the red-black tile shape of later mixed CG solves (rows per tile 2/4/8/10/12/13,
+        """Force the red-black tile shape of later mixed CG solves (rows per tile 2/4/8/9/10/12/13,
         planes per tile, 0 = size rule; fused-pass prefetch 2/3); ty = 0 restores the
         tuned/automatic choice.  Results do not depend on it."""
         check(lib().fast_ctx_set_rb_shape(self._h, ty, tz, ahead), "fast_ctx_set_rb_shape", self._h)

@@ -128,7 +128,7 @@ size_t fast_ctx_device_bytes(const fast_ctx* ctx);
  * the autotuner (fast_amd/autotune.py).  No reference counterpart. */
 int fast_ctx_set_tile_shape(fast_ctx* ctx, int tx, int ty, int tz);
 /* Force the tile shape of subsequent red-black (mixed CG) solves: ty rows of 64 slots per
- * tile (2, 4, 8, 10, 12 or 13), tz planes per tile (1..256; 0 = the size rule) and the fused Y+Q pass's
+ * tile (2, 4, 8, 9, 10, 12 or 13), tz planes per tile (1..256; 0 = the size rule) and the fused Y+Q pass's
  * prefetch distance in planes (2 or 3).  ty = 0 restores the automatic choice (the autotuned
  * "rb_" table entries, else 4 rows, the size rule and 2).  Used by the autotuner
  * (fast_amd/autotune.py).  No reference counterpart. */

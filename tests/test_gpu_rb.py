@@ -168,7 +168,7 @@ def test_compacted_batch_equals_solo(ctx):
 
 
 def test_tile_shapes_bit_identical(ctx):
-    """Rows per tile (2, 4, 8, 10, 12, 13), planes per tile and the fused pass's prefetch distance change
+    """Rows per tile (2, 4, 8, 9, 10, 12, 13), planes per tile and the fused pass's prefetch distance change
     only the launch geometry: every dot product is rounded once from a double-double sum, so
     tau, iterations and the field are identical, on a cube and on shapes ragged in every axis."""
     cases = [F.generate_spheres(48, 5.0, 0.55, 2), rand_mask((21, 13, 29), 0.45, 11), rand_mask((66, 9, 5), 0.35, 12)]
@@ -177,8 +177,8 @@ def test_tile_shapes_bit_identical(ctx):
         p = T.make_params(4, CG, "mixed", 1e-9, 10000)
         ref, cref = vol.solve(p, conc=True)
         assert ref.status == 0
-        for ty, tz, ahead in ((2, 0, 3), (8, 0, 3), (4, 0, 2), (2, 3, 2), (8, 5, 3), (4, 32, 3), (10, 0, 2), (13, 7, 3),
-                              (12, 0, 2), (12, 4, 3)):
+        for ty, tz, ahead in ((2, 0, 3), (8, 0, 3), (4, 0, 2), (2, 3, 2), (8, 5, 3), (4, 32, 3), (10, 0, 2),
+                              (13, 7, 3), (9, 0, 2), (9, 6, 3), (12, 0, 2), (12, 4, 3)):
             ctx.set_rb_shape(ty, tz, ahead)
             got, c = vol.solve(p, conc=True)
             ctx.set_rb_shape(0)
