@@ -494,8 +494,10 @@ def main() -> int:
         "dtype": "f32 storage, f64 arithmetic/reductions/solution",
         "data": "synthetic",
         "config": {
-            "workload": f"{n}^3 ellipsoids a=12 r=0.35 solid 0.65 seed 0; tol 1e-6; x,y,z solved as one "
-                        "batched mixed-precision CG solve (red-black Schur complement) per step",
+            "workload": f"{n}^3 ellipsoids a=12 r=0.35 solid 0.65 seed 0; tol 1e-6; x,y,z submitted as one "
+                        "batch of mixed-precision CG solves (red-black Schur complement) per step; the library "
+                        "runs a batch of volumes this large as consecutive one-volume launches (each volume's "
+                        "state is about the size of the 256 MB Infinity Cache)",
             "volume": [n, n, n], "solves_per_step": len(DIRS), "parallelism": "single GPU",
             "red_black_tile": dict(zip(("rows", "planes", "prefetch_planes"), ctx.rb_shape((n, n, n)))),
             "red_black_tile_source": "autotuned table (fast_amd/tuning_gfx950.txt, rb_ keys) or the size rule",
