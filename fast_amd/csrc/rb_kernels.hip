@@ -101,7 +101,9 @@ struct RbIn {
 //              last iteration's step (xpend), x += alpha p_old with that iteration's alpha
 //   Y (black): Y = D_b^-1 (A_br p)
 //   Q (red):   q = D_r p - A_rb Y = S p; partial p.q
-//   U (red):   r -= alpha q; partials r.r and r.(D^-1 r)
+//   U (red):   r -= alpha q; partials r.r and r.(D^-1 r).  alpha = (r.z) / (p.q) comes from a
+//              finalize launch between Q and U, or (RbLaunch::fold_alpha, one-volume launch
+//              sets) every U block reduces the Q pass's partials itself, which saves that launch
 // x += alpha p is deferred from U to the next P pass, which loads p_old anyway: U then reads
 // r, q and the code (13 B per red voxel instead of 25) and P also reads and writes x (21
 // instead of 13).  x takes the same values and roundings; the update pass adds a pending
@@ -116,14 +118,15 @@ __global__ __launch_bounds__(kRbTX* TY) void k_rb_it(RbLaunch L) {
     const int v = L.tile_vol[slot];
     const int tile = L.tile_id ? L.tile_id[slot] : slot;   // the tile's index in the batch
     const RbDesc d = L.vol[v];   // by value: kept in registers across the stores
-    const VolState<double>* vs = L.st + v;
+    VolState<double>* vs = L.st + v;
     if (vs->status != ST_RUNNING) return;
     __shared__ RbTabs tabs;
     const Coef<double> c = L.coef[v];
     rb_load_tables<KIND == RB_Q ? 0 : 1, kRbTX * TY>(L, v, c, tabs);   // Q reads D; P, Y and U read D^-1
     const RbPos ps = rb_pos<TY>(d, tile - d.tile_base);
     constexpr int kColour = (KIND == RB_Y) ? 1 : 0;
-    const double alpha = vs->alpha, beta = vs->beta;
+    double alpha = vs->alpha;   // U: replaced by this iteration's alpha below
+    const double beta = vs->beta;
     const bool first = vs->first != 0;
     const bool xpend = KIND == RB_P && vs->xpend != 0;   // uniform over the volume
     const bool odd = (vs->phase & 1) != 0;
@@ -194,6 +197,25 @@ __global__ __launch_bounds__(kRbTX* TY) void k_rb_it(RbLaunch L) {
     RbIn in[R];
 #pragma unroll
     for (int i = 0; i < AH; ++i) load(ps.z0 + i, in[i]);
+    if (KIND == RB_U && L.fold_alpha) {
+        // alpha = (r.z) / (p.q) from the Q pass's tile partials of the volume, reduced by every
+        // block alike while its first planes load (a double-double sum rounds to the same value
+        // for any reduction tree), as the finalize would; the block of the volume's first tile
+        // stores it for the next P pass and the update pass, and a non-finite alpha starts the
+        // volume afresh
+        __shared__ double pq;
+        double tot[1];
+        reduce_volume<1, kRbTX * TY>(L.part, d.tile_base, d.ntiles, tot);
+        if (threadIdx.x == 0) pq = tot[0];
+        __syncthreads();
+        alpha = vs->absnew / pq;
+        if (tile == d.tile_base && threadIdx.x == 0) {
+            vs->alpha = alpha;
+            vs->xpend = 0;   // the P pass of this iteration has added the pending step
+            if (!isfinite(alpha)) vs->status = ST_FRESH;
+        }
+        if (!isfinite(alpha)) return;
+    }
     for (int z = ps.z0; z < ps.z1; z += R) {
 #pragma unroll
         for (int j = 0; j < R; ++j) {
@@ -202,7 +224,9 @@ __global__ __launch_bounds__(kRbTX* TY) void k_rb_it(RbLaunch L) {
         }
     }
     if (KIND == RB_Q) block_reduce_store<AccDD, 1, kRbTX * TY>(reinterpret_cast<AccDD(&)[1]>(acc), L.part + (long long)tile * kPart);
-    if (KIND == RB_U) block_reduce_store<AccDD, 2, kRbTX * TY>(acc, L.part + (long long)tile * kPart);
+    // U's partials go after the Q pass's p.q in the tile's slots: blocks still reducing p.q
+    // read the first two
+    if (KIND == RB_U) block_reduce_store<AccDD, 2, kRbTX * TY>(acc, L.part + (long long)tile * kPart + 2);
 }
 
 // Q with the black Y planes staged in LDS: a block owns kRbTX slots x TY rows of red
@@ -660,7 +684,8 @@ __global__ __launch_bounds__(kFinThreads) void k_rb_fin(RbLaunch L) {
     if (s->status != ST_RUNNING) return;
     constexpr int NP = (FIN == F_CG2) ? 2 : 1;
     double tot[NP];
-    reduce_volume<NP, kFinThreads>(L.part, d.tile_base, d.ntiles, tot);
+    // the U pass's partials sit after the Q pass's p.q in each tile's slots
+    reduce_volume<NP, kFinThreads>(L.part + (FIN == F_CG2 ? 2 : 0), d.tile_base, d.ntiles, tot);
     if (threadIdx.x == 0) {
         double ta[3] = {0.0, 0.0, 0.0};
 #pragma unroll
@@ -782,7 +807,7 @@ hipError_t rb_iter(const RbLaunch& L, hipStream_t s) {
     rb_launch_it<RB_P>(L, s);
     if (L.staged < 2) rb_launch_it<RB_Y>(L, s);   // the fused form computes Y itself
     rb_launch_q(L, s);
-    hipLaunchKernelGGL((k_rb_fin<F_CG1>), dim3(L.nvol), dim3(kFinThreads), 0, s, L);
+    if (!L.fold_alpha) hipLaunchKernelGGL((k_rb_fin<F_CG1>), dim3(L.nvol), dim3(kFinThreads), 0, s, L);
     rb_launch_it<RB_U>(L, s);
     hipLaunchKernelGGL((k_rb_fin<F_CG2>), dim3(L.nvol), dim3(kFinThreads), 0, s, L);
     return hipGetLastError();

@@ -271,6 +271,7 @@ int rebuild_rb(fast_server* s, const std::vector<Req*>& adm) {
     L.ntiles = tiles;
     L.nvol = s->max_active;
     L.staged = rb_mode();
+    L.fold_alpha = 0;
     L.ty = s->rbs.ty;
     L.ahead = s->rbs.ahead;
     L.vol = s->d_rb;

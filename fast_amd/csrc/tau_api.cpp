@@ -1059,6 +1059,7 @@ static int prepare_rb(RbBatch& B) {
     L.ntiles = tiles;
     L.nvol = nv;
     L.staged = rb_mode();
+    L.fold_alpha = nv == 1 ? 1 : 0;   // one volume: no finalize launch between Q and U
     L.ty = bs.ty;
     L.ahead = bs.ahead;
     L.vol = (const RbDesc*)(m + o_rb);

@@ -190,6 +190,8 @@ struct RbLaunch {
     int ahead;        // fused Y+Q pass: planes its loads run ahead (2 or 3)
     int staged;       // 2: Y and Q fused (no Y vector); 1: Q pass with the Y planes staged in LDS; 0: six
                       // Y loads per voxel
+    int fold_alpha;   // 1: the U pass reduces the Q pass's partials into alpha itself (one-volume
+                      // launch sets); 0: a finalize launch between Q and U does
     const RbDesc* vol;
     const int* tile_vol;   // volume of each launched tile
     const int* tile_id;    // its index in the batch (tile_base + position); null: the launch index
