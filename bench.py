@@ -477,6 +477,9 @@ def main() -> int:
             roofline["note"] = ("the fused Y+Q pass is bound by instruction issue, not HBM: it moves 10 B per red "
                                 "voxel where the separate Y and Q passes it replaces move 30 B; rate on those 30 B: "
                                 f"{dram['fused_yq_unfused_equivalent_GBs']:.0f} GB/s")
+            eq = dram["fused_yq_unfused_equivalent_GBs"]
+            roofline["unfused_equivalent"] = {"GBs": eq, "frac": eq / HBM_PEAK_GBS,
+                                              "frac_of_measured_ceiling": eq / ceil if ceil else None}
     else:
         roofline = dict(at_n, bound="hbm", peak=HBM_PEAK_GBS, unit="GB/s",
                         timing="HIP events on the solve stream around 20 back-to-back launches of the kernel")
