@@ -1059,7 +1059,7 @@ static int prepare_rb(RbBatch& B) {
     L.ntiles = tiles;
     L.nvol = nv;
     L.staged = rb_mode();
-    L.fold_alpha = nv == 1 ? 1 : 0;   // one volume: no finalize launch between Q and U
+    L.fold_alpha = (nv == 1 && tiles <= kRbFoldMaxTiles) ? 1 : 0;
     L.ty = bs.ty;
     L.ahead = bs.ahead;
     L.vol = (const RbDesc*)(m + o_rb);
