@@ -400,13 +400,13 @@ def main() -> int:
     fused = os.environ.get("FAST_TAU_RB_STAGED", "2") == "2"
     if fused:   # Y computed inside the Q pass: p, red and black codes in, q out
         kernels = {
-            "k_rb_it<0> (P, red: p = D^-1 r + beta p, x += alpha p_old)": (rb_pass[0], (4 + 4 + 1 + 4 + 2 * 4) * n_r),
+            "k_rb_it<0> (P, red: p = D^-1 r + beta p, x += a0 p_prev + a1 p_old)": (rb_pass[0], (4 + 4 + 1 + 4 + 4 + 2 * 4) * n_r),
             "k_rb_yq (Y+Q fused, red: q = S p = D_r p - A_rb D_b^-1 A_br p, p.q)": (rb_pass[2], (4 + 1 + 4) * n_r + 1 * n_b),
             "k_rb_it<3> (U, red: r -= alpha q, r.r, r.D^-1 r)": (rb_pass[3], (2 * 4 + 1 + 4) * n_r),
         }
     else:
         kernels = {
-            "k_rb_it<0> (P, red: p = D^-1 r + beta p, x += alpha p_old)": (rb_pass[0], (4 + 4 + 1 + 4 + 2 * 4) * n_r),
+            "k_rb_it<0> (P, red: p = D^-1 r + beta p, x += a0 p_prev + a1 p_old)": (rb_pass[0], (4 + 4 + 1 + 4 + 4 + 2 * 4) * n_r),
             "k_rb_it<1> (Y, black: Y = D_b^-1 A_br p)": (rb_pass[1], (1 + 8) * n_b + 4 * n_r),
             "k_rb_q (Q, red: q = S p = D_r p - A_rb Y, p.q)": (rb_pass[2], (1 + 4 + 4) * n_r + 8 * n_b),
             "k_rb_it<3> (U, red: r -= alpha q, r.r, r.D^-1 r)": (rb_pass[3], (2 * 4 + 1 + 4) * n_r),
@@ -417,7 +417,9 @@ def main() -> int:
     others = {k: {"avg_launch_ms": v[0], "bytes_per_launch": v[1], "achieved_GBs": v[1] / (v[0] * 1e-3) / 1e9,
                   "traffic": measured_traffic(k.split(" ")[0], n)}
               for k, v in kernels.items()}
-    iter_bytes = sum(v[1] for v in kernels.values())
+    # P is timed in the iterations that add the two pending x steps (25 B per red voxel); the
+    # other half of the iterations' P passes move 13 B, so an iteration averages 6 B less
+    iter_bytes = sum(v[1] for v in kernels.values()) - 6 * n_r
     # the red-black state of one 256^3 volume (~260 MB) is about the size of the 256 MB
     # Infinity Cache, so part of the 256^3 traffic may be served from it; the same passes on
     # the 512^3 needle volume (~3 GB of state) are DRAM-bound
@@ -432,9 +434,9 @@ def main() -> int:
         nb = 512 ** 3
         nr5 = nb // 2
         if fused:
-            b5 = {0: ("k_rb_it<0> (P)", 21 * nr5), 2: ("k_rb_yq (Y+Q)", 10 * nr5), 3: ("k_rb_it<3> (U)", 13 * nr5)}
+            b5 = {0: ("k_rb_it<0> (P)", 25 * nr5), 2: ("k_rb_yq (Y+Q)", 10 * nr5), 3: ("k_rb_it<3> (U)", 13 * nr5)}
         else:
-            b5 = {0: ("k_rb_it<0> (P)", 21 * nr5), 1: ("k_rb_it<1> (Y)", 13 * nr5), 2: ("k_rb_q (Q)", 17 * nr5),
+            b5 = {0: ("k_rb_it<0> (P)", 25 * nr5), 1: ("k_rb_it<1> (Y)", 13 * nr5), 2: ("k_rb_q (Q)", 17 * nr5),
                   3: ("k_rb_it<3> (U)", 13 * nr5)}
         dram = {"volume": "512^3 needles a=12 r=0.1 solid 0.55 seed 0, X_POS",
                 "red_black_tile": dict(zip(("rows", "planes", "prefetch_planes"), ctx.rb_shape((512, 512, 512)))),
@@ -442,7 +444,7 @@ def main() -> int:
                                 "achieved_GBs": b / (rb[i] * 1e-3) / 1e9, "frac": b / (rb[i] * 1e-3) / 8e12,
                                 "traffic": measured_traffic(k.split(" ")[0], 512)}
                             for i, (k, b) in b5.items()},
-                "iteration_ms": rb[4], "iteration_GBs": sum(b for _, b in b5.values()) / (rb[4] * 1e-3) / 1e9}
+                "iteration_ms": rb[4], "iteration_GBs": (sum(b for _, b in b5.values()) - 6 * nr5) / (rb[4] * 1e-3) / 1e9}
         log("[bench] streaming kernels over 3 x 2 GiB (measured HBM ceiling)")
         dram["hbm_ceiling"] = hbm_ceiling(ctx, lib)
         if fused:   # the separate Y and Q passes the fused one replaces would move 30 B per red voxel

@@ -97,17 +97,19 @@ struct RbIn {
 };
 
 // The iteration passes, one thread per slot of a row, marching tz planes:
-//   P (red):   p = D^-1 r (first iteration) or D^-1 r + beta p_old; and, while x lacks the
-//              last iteration's step (xpend), x += alpha p_old with that iteration's alpha
+//   P (red):   p = D^-1 r (first iteration) or D^-1 r + beta p_old; and, when x lacks the
+//              last two iterations' steps (xpend == 2), x += a0 p_prev + a1 p_old, p_prev being
+//              the older p that this pass overwrites
 //   Y (black): Y = D_b^-1 (A_br p)
 //   Q (red):   q = D_r p - A_rb Y = S p; partial p.q
 //   U (red):   r -= alpha q; partials r.r and r.(D^-1 r).  alpha = (r.z) / (p.q) comes from a
 //              finalize launch between Q and U, or (RbLaunch::fold_alpha, one-volume launch
 //              sets) every U block reduces the Q pass's partials itself, which saves that launch
-// x += alpha p is deferred from U to the next P pass, which loads p_old anyway: U then reads
-// r, q and the code (13 B per red voxel instead of 25) and P also reads and writes x (21
-// instead of 13).  x takes the same values and roundings; the update pass adds a pending
-// step before it folds x into x64.
+// x += alpha p is deferred from U: every second P pass adds the last two iterations' steps
+// (it loads p_old anyway, and p_prev is still in the buffer it overwrites), so U reads r, q
+// and the code (13 B per red voxel instead of 25) and P reads and writes x every other
+// iteration (13 or 25 B, 19 on average).  x rounds once per two steps; the update pass adds
+// whatever is pending before it folds x into x64.
 // Loads are issued AH planes ahead of use into AH + 1 register sets used in rotation (the
 // loop is unrolled by AH + 1), all unconditional: an invalid point reads the first interior
 // slot, and ghosts make neighbour reads safe, so each plane waits only for its own loads.
@@ -128,7 +130,8 @@ __global__ __launch_bounds__(kRbTX* TY) void k_rb_it(RbLaunch L) {
     double alpha = vs->alpha;   // U: replaced by this iteration's alpha below
     const double beta = vs->beta;
     const bool first = vs->first != 0;
-    const bool xpend = KIND == RB_P && vs->xpend != 0;   // uniform over the volume
+    const bool xpend = KIND == RB_P && vs->xpend == 2;   // uniform over the volume
+    const double xa0 = vs->xalpha[0], xa1 = vs->xalpha[1];
     const bool odd = (vs->phase & 1) != 0;
     const float* p_old = L.p[odd ? 1 : 0] + d.off_c;
     float* p_new = L.p[odd ? 0 : 1] + d.off_c;
@@ -150,7 +153,10 @@ __global__ __launch_bounds__(kRbTX* TY) void k_rb_it(RbLaunch L) {
         if (KIND == RB_P) {
             in.a = ld(Rr, ci);
             in.b = ld(p_old, ci);   // unused on the first iteration
-            if (xpend) in.c = ld(X, ci);
+            if (xpend) {
+                in.c = ld(X, ci);
+                in.e = ld(p_new, ci);   // p two iterations back, about to be overwritten
+            }
         } else if (KIND == RB_Y) {
             rb_nbrs(p_new, d, ci, s, in.u);
         } else if (KIND == RB_Q) {
@@ -175,7 +181,7 @@ __global__ __launch_bounds__(kRbTX* TY) void k_rb_it(RbLaunch L) {
             const double z_ = di.iv * (double)in.a;
             const double pn = first ? z_ : z_ + beta * (double)in.b;
             st(p_new, ci, (float)pn);
-            if (xpend) st(X, ci, (float)((double)in.c + alpha * (double)in.b));
+            if (xpend) st(X, ci, (float)(((double)in.c + xa0 * (double)in.e) + xa1 * (double)in.b));
         } else if (KIND == RB_Y) {
             st(Y, ci, rb_sum(tabs.ftab, code, in.u) * di.iv);
         } else if (KIND == RB_Q) {
@@ -211,7 +217,7 @@ __global__ __launch_bounds__(kRbTX* TY) void k_rb_it(RbLaunch L) {
         alpha = vs->absnew / pq;
         if (tile == d.tile_base && threadIdx.x == 0) {
             vs->alpha = alpha;
-            vs->xpend = 0;   // the P pass of this iteration has added the pending step
+            if (vs->xpend == 2) vs->xpend = 0;   // the P pass of this iteration added both steps
             if (!isfinite(alpha)) vs->status = ST_FRESH;
         }
         if (!isfinite(alpha)) return;
@@ -570,11 +576,12 @@ __global__ __launch_bounds__(kRbTX* TY) void k_rb(RbLaunch L) {
     constexpr int kColour = (KIND == RB_Y64) ? 1 : 0;
     const bool first_res = vs->refinements == 0;   // x64 not yet set: use x0
     const bool keep = vs->refinements > 0 && !vs->discard;
-    // x lacks alpha p of the last iteration (p is the newer buffer: the iteration's finalize
-    // has advanced the phase), added with the P pass's rounding
-    const bool xpend = KIND == RB_UPD && vs->xpend != 0;
-    const double alpha = vs->alpha;
+    // x lacks the steps of the last xpend iterations: the newest in the buffer the next P pass
+    // reads as p_old, an older one in the other buffer; added in the P pass's order
+    const int xpend = KIND == RB_UPD ? vs->xpend : 0;
+    const double xa0 = vs->xalpha[0], xa1 = vs->xalpha[1];
     const float* p_last = L.p[(vs->phase & 1) ? 1 : 0] + d.off_c;
+    const float* p_prev = L.p[(vs->phase & 1) ? 0 : 1] + d.off_c;
     const int nprim = d.primary == 0 ? d.nx : (d.primary == 1 ? d.ny : d.nz);
     float* X = L.x + d.off_c;
     float* Rr = L.r + d.off_c;
@@ -605,7 +612,9 @@ __global__ __launch_bounds__(kRbTX* TY) void k_rb(RbLaunch L) {
             if (x >= d.nx) continue;
             if (KIND == RB_UPD) {
                 if (keep) {
-                    const float xv = xpend ? (float)((double)X[ci] + alpha * (double)p_last[ci]) : X[ci];
+                    const float xv = xpend == 2   ? (float)(((double)X[ci] + xa0 * (double)p_prev[ci]) + xa1 * (double)p_last[ci])
+                                     : xpend == 1 ? (float)((double)X[ci] + xa0 * (double)p_last[ci])
+                                                  : X[ci];
                     X64[ci] = X64[ci] + (double)xv;
                 }
                 X[ci] = 0.0f;
@@ -690,8 +699,8 @@ __global__ __launch_bounds__(kFinThreads) void k_rb_fin(RbLaunch L) {
         double ta[3] = {0.0, 0.0, 0.0};
 #pragma unroll
         for (int k = 0; k < NP; ++k) ta[k] = tot[k];
-        // after Q: the P pass has added the pending step; after U: x lacks this iteration's
-        s->xpend = FIN == F_CG2 ? 1 : 0;
+        if (FIN == F_CG1 && s->xpend == 2) s->xpend = 0;   // the P pass added both steps
+        if (FIN == F_CG2) s->xalpha[s->xpend++] = s->alpha;   // x lacks this iteration's step
         fin_update<double, FIN>(s, ta);
     }
 }

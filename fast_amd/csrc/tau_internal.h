@@ -58,8 +58,9 @@ struct VolState {
     int discard;                   // mixed: drop d (non-finite) at the next update
     int reshadow;                  // mixed: the next update also resets r0 = r (Eigen's restart)
     int solver;                    // mixed: 0 = BiCGSTAB, 1 = CG
-    int xpend;                     // red-black CG: x still lacks alpha p of the last iteration
-                                   // (added by the next P pass, or by the update pass)
+    int xpend;                     // red-black CG: steps alpha p that x still lacks (0..2), the
+                                   // last xpend iterations'; the P pass adds two, the update any
+    double xalpha[2];              // their alphas, oldest first
 };
 
 // Vectors of one (batched) solve, concatenated over its volumes (element offset VolDesc::off).
