@@ -207,8 +207,8 @@ __global__ __launch_bounds__(kRbTX* TY) void k_rb_it(RbLaunch L) {
         // alpha = (r.z) / (p.q) from the Q pass's tile partials of the volume, reduced by every
         // block alike while its first planes load (a double-double sum rounds to the same value
         // for any reduction tree), as the finalize would; the block of the volume's first tile
-        // stores it for the next P pass and the update pass, and a non-finite alpha starts the
-        // volume afresh
+        // stores it (the finalize after U records it with x's pending step), and a non-finite
+        // alpha starts the volume afresh
         __shared__ double pq;
         double tot[1];
         reduce_volume<1, kRbTX * TY>(L.part, d.tile_base, d.ntiles, tot);
@@ -548,7 +548,7 @@ void k_rb_yq(RbLaunch L) {
 
 // The passes outside the Krylov loop, one thread per slot of a row, marching tz planes:
 //   CODE:    per-colour stencil codes from the full-grid codes (set-up)
-//   UPD:     x64 += x (kept unless discarded; a pending x += alpha p added first), x = 0
+//   UPD:     x64 += x (kept unless discarded; x's pending steps alpha p added first), x = 0
 //                                                                          [update / fresh start]
 //   Y64:     black x64b = D_b^-1 (b_b - A_br x64), partial |b_b|^2       [update / fresh start]
 //   R64:     red r = b_S - S x64 = (b_r - D_r x64) - A_rb x64b in FP64, stored in FP32;
